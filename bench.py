@@ -1,0 +1,173 @@
+"""Throughput of the PIQMC sweep on one MI355X.
+
+Workload (one "step" = one anneal of the whole batch):
+    2-D +-J spin glass L=32 (N=1024), periodic, J=+-1, h=0; P=64 Trotter
+    slices, T=1/64 (P*T=1); 4096 independent replicas; Gamma linear
+    3.0 -> 1e-8 over 500 steps x 1 sweep; int8 spins in HBM between calls.
+Headline: spin-flip attempts/s of the parallel (graph-colored,
+parity-alternated) sweep.  Also reported: the bit-exact replay sweep on the
+same workload (fewer field steps; the rate is size-stable), the roofline of
+the dominant kernel from HIP events, and the CPU restatement of the reference
+QuantumAnneal (one thread) and of its OpenMP variant timed on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-replay]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pathintegral-qmc_amd"))
+
+LDS_PEAK_GBS = 128 * 256 * 2.4  # 128 B/clk/CU x 256 CU x 2.4 GHz (nominal)
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_ATTEMPT = 8  # int8 spin traffic of one 2-D attempt: self + 4 + 2 Trotter + write
+
+
+def cpu_baseline(nbs, N, P, T, sched, reps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+
+    # reference QuantumAnneal restated in C, one thread, F-ordered float64 confs
+    tot, att = 0.0, 0
+    for r in range(reps):
+        rng = np.random.RandomState(1000 + r)
+        sv = np.array([2 * rng.randint(2) - 1 for _ in range(N)], np.float64)
+        confs = np.tile(sv, (P, 1)).T
+        perms = O.MT(2000 + r).perm_stream(N, len(sched))
+        t0 = time.perf_counter()
+        O.qa_compat(sched, 1, P, T, N, confs, nbs, perms, O.Type3(3000 + r))
+        tot += time.perf_counter() - t0
+        att += N * P * len(sched)
+    single = att / tot
+    # QuantumAnneal_parallel restated (OpenMP guided, libc rand), all host threads
+    nthr = os.cpu_count() or 1
+    sub = sched[:: max(1, len(sched) // 50)]
+    rng = np.random.RandomState(7)
+    sv = np.array([2 * rng.randint(2) - 1 for _ in range(N)], np.float64)
+    confs = np.tile(sv, (P, 1)).T
+    t0 = time.perf_counter()
+    O.qa_parallel(sub, 1, P, T, N, confs, nbs, nthr)
+    omp = N * P * len(sub) / (time.perf_counter() - t0)
+    return single, omp, nthr, "%d replica(s) x %d field steps of the 1024-spin workload (%d attempts)" % (
+        reps, len(sched), att)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--replicas", type=int, default=4096)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-replay", action="store_true")
+    ap.add_argument("--replay-steps", type=int, default=25)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the sweep shards replicas with no exchange, so "
+              "--gpus N > 1 is not run here", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from piqmc import device, instances
+
+    dev = torch.device("cuda:0")
+    L, P, T, S, R = 32, 64, 1.0 / 64, 500, args.replicas
+    N, r, c, v = instances.lattice_pm_j(L, 2, np.random.RandomState(2024))
+    nbs = instances.neighbors(N, r, c, v)
+    pb = device.Problem(nbs)
+    sched = np.linspace(3.0, 1e-8, S)
+    g = torch.Generator(device="cpu").manual_seed(0)
+    init = (torch.randint(0, 2, (R, P, N), generator=g, dtype=torch.int8) * 2 - 1).to(dev)
+    confs = torch.empty_like(init)
+    attempts = N * P * S * R
+
+    def step():
+        confs.copy_(init)
+        pb.qa_colored(confs, sched, 1, T, 12345, block=args.block)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        kms.append(device.last_kernel_ms())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    value = attempts * args.steps / dt
+    kavg = sum(kms) / len(kms) / 1e3
+    k_rate = attempts / kavg
+    achieved = BYTES_PER_ATTEMPT * attempts / kavg / 1e9
+    E = pb.slice_energies(confs)
+    e_min = float(E.min(1).values.mean()) / N
+
+    out = {
+        "metric": "spin-flip attempts/s (1 GPU)",
+        "value": value,
+        "unit": "attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8 spins (1-bit packed in LDS), fp64 acceptance",
+        "data": "synthetic: seeded +-J instance and random +-1 initial spins",
+        "config": {"workload": "2-D +-J glass L=32 N=1024 periodic, P=64, T=1/64, 4096 replicas, "
+                               "Gamma 3.0->1e-8 over 500 steps x 1 sweep (parallel sweep)",
+                   "replicas": R, "nspins": N, "slices": P, "field_steps": S, "sweeps_per_step": 1},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "colored_kernel<uint32,+-1 path>",
+                     "kernel_ms": kavg * 1e3, "kernel_attempts_per_s": k_rate,
+                     "bytes_per_attempt": BYTES_PER_ATTEMPT,
+                     "lds_frac": achieved / LDS_PEAK_GBS},
+        "mean_min_slice_energy_per_spin": e_min,
+    }
+
+    if not args.no_replay:
+        # bit-exact replay on the same instance, all replicas, fewer field steps
+        Sr = args.replay_steps
+        sr = np.linspace(3.0, 1e-8, Sr)
+        perms = torch.from_numpy(device.perm_streams(np.arange(R, dtype=np.uint32) + 1, N, Sr)
+                                 .view(np.int16)).to(dev)
+        rs0 = torch.from_numpy(device.rand_states(np.arange(R) + 7).view(np.int32)).to(dev)
+        rs = rs0.clone()
+        c2 = init.clone()
+        pb.qa_replay(c2, sr, 1, T, perms, rs)          # warm-up
+        torch.cuda.synchronize()
+        c2.copy_(init)
+        rs.copy_(rs0)
+        t0 = time.perf_counter()
+        draws, ties = pb.qa_replay(c2, sr, 1, T, perms, rs)
+        torch.cuda.synchronize()
+        rdt = time.perf_counter() - t0
+        rk = device.last_kernel_ms() / 1e3
+        ra = N * P * Sr * R
+        out["replay"] = {"attempts_per_s": ra / rdt, "kernel_attempts_per_s": ra / rk,
+                         "kernel_ms": rk * 1e3, "field_steps": Sr, "replicas": R,
+                         "near_ties": int(ties.sum())}
+        del perms
+
+    if not args.no_cpu:
+        single, omp, nthr, sample = cpu_baseline(nbs, N, P, np.float32(T), sched, reps=2)
+        out["cpu_baseline"] = {"value": single, "unit": "attempts/s", "cores": 1, "kind": "port",
+                               "sample": "reference QuantumAnneal restated in C (oracle), " + sample}
+        out["cpu_openmp"] = {"value": omp, "unit": "attempts/s", "cores": nthr, "kind": "port",
+                             "sample": "QuantumAnneal_parallel restated (OpenMP guided, libc rand)"}
+        out["speedup_vs_cpu"] = value / single
+        if "replay" in out:
+            out["replay"]["speedup_vs_cpu"] = out["replay"]["attempts_per_s"] / single
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

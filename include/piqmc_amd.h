@@ -1,0 +1,156 @@
+/*
+ * piqmc_amd.h -- C ABI of libpiqmc_amd.so, the MI355X (gfx950) implementation
+ * of piqmc's path-integral quantum-annealing sweep.
+ *
+ * The reference (hadsed/pathintegral-qmc) exposes its sweep only as Cython
+ * cpdef functions inside CPython extension modules; there is no C symbol to
+ * bind.  The entry points below are what a Cython/ctypes shim for those
+ * functions binds instead.  Each cites the reference interface it replaces.
+ *
+ * Conventions
+ *  - Every function returns a status (PIQMC_OK or one of the codes below);
+ *    piqmc_last_error() returns a message for the calling thread.
+ *  - "host" pointers are ordinary CPU memory, "dev" pointers are HIP device
+ *    memory of the current device; `stream` is a hipStream_t (NULL = default).
+ *  - Spins are +1/-1.  Device spin arrays are int8.
+ *  - Neighbour tables are tools.GenerateNeighbors arrays: float64
+ *    [nspins][maxnb][2] holding (neighbour index, coupling); an entry whose
+ *    index equals its row is a local field (piqmc/qmc.pyx:107-109).
+ */
+#ifndef PIQMC_AMD_H
+#define PIQMC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIQMC_OK 0
+#define PIQMC_EINVAL 1    /* bad shape / value      -> ValueError        */
+#define PIQMC_EINDEX 2    /* neighbour row overflow -> IndexError        */
+#define PIQMC_EZERODIV 3  /* slices*temp == 0 or temp == 0 -> ZeroDivisionError */
+#define PIQMC_EHIP 4      /* HIP runtime / device error -> RuntimeError  */
+#define PIQMC_ENOMEM 5    /* allocation failure     -> MemoryError       */
+
+const char *piqmc_last_error(void);
+int piqmc_version(void);
+/* Device time of the main sweep kernel of the last replay / parallel call made
+ * from this host thread (HIP events on the call's stream; waits for it).
+ * No reference counterpart: the reference has no device work to time. */
+int piqmc_last_kernel_ms(float *ms);
+
+/* ------------------------------------------------------------------ */
+/* Problem handle: a neighbour table resident in HBM plus its graph   */
+/* coloring (for the parallel sweep).                                 */
+/* Replaces the `nbs` argument conversion of QuantumAnneal            */
+/* (piqmc/qmc.pyx:36, memoryview unpack piqmc/qmc.c:2655).            */
+/* ------------------------------------------------------------------ */
+typedef struct piqmc_problem piqmc_problem;
+
+int piqmc_problem_create(int nspins, int maxnb, const double *nbs_host,
+                         piqmc_problem **out);
+int piqmc_problem_destroy(piqmc_problem *pb);
+/* ncolors: spatial colors found; pm_j: 1 if every coupling is +-1 between
+ * distinct spins (bit-sliced path), zmax: max couplings per spin. */
+int piqmc_problem_info(const piqmc_problem *pb, int *nspins, int *maxnb,
+                       int *ncolors, int *pm_j, int *zmax);
+/* Replace the spatial coloring (color_host[nspins], values 0..ncolors-1).
+ * Rejected unless no two coupled spins share a color. */
+int piqmc_problem_set_coloring(piqmc_problem *pb, int ncolors, const int32_t *color_host);
+/* The coloring in use (color_host[nspins]). */
+int piqmc_problem_get_coloring(const piqmc_problem *pb, int32_t *color_host);
+
+/* ------------------------------------------------------------------ */
+/* Random streams of the reference, for replay.                       */
+/* ------------------------------------------------------------------ */
+/* glibc srand(seed) image of the TYPE_3 state used by libc rand()
+ * (piqmc/qmc.pyx:22,132): state[0..30] words, state[31] = front index. */
+int piqmc_rand_seed(uint32_t seed, uint32_t *state32_host);
+/* Advance a TYPE_3 state by n draws on the host. */
+int piqmc_rand_advance(uint32_t *state32_host, uint64_t n);
+/* numpy RandomState(seed) permutation streams as QuantumAnneal draws them
+ * (piqmc/qmc.pyx:90-91,136): stream r, permutation g, position t lands at
+ * perms[(g*N + t)*R + r].  Stream r starts from RandomState(seeds[r]). */
+int piqmc_perm_streams(const uint32_t *seeds_host, int R, int N, int G,
+                       uint16_t *perms_host, int nthreads);
+
+/* ------------------------------------------------------------------ */
+/* Replay sweeps (bit-exact with the reference).                      */
+/* ------------------------------------------------------------------ */
+/* qmc.QuantumAnneal (piqmc/qmc.pyx:30-136) for R independent replicas.
+ *  sched_host [S] transverse fields; M sweeps per field; P slices; temp.
+ *  confs_dev  int8 [R][P][N] in/out.
+ *  perms_dev  uint16 [S*M][N][R]: the permutation used by sweep g.
+ *  rand_dev   uint32 [R][32] TYPE_3 states in/out (see piqmc_rand_seed).
+ *  draws_dev  uint64 [R] out: rand() calls made.
+ *  ties_dev   uint32 [R] out: acceptance tests where the device exp() and a
+ *             correctly rounded exp could disagree (|exp - u| <= 2^-50 exp);
+ *             zero means the run is bit-exact with the libm reference. */
+int piqmc_qa_replay(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
+                    const double *sched_host, int8_t *confs_dev, const uint16_t *perms_dev,
+                    uint32_t *rand_dev, uint64_t *draws_dev, uint32_t *ties_dev,
+                    void *stream);
+/* sa.Anneal (piqmc/sa.pyx:50-120) for R replicas; svec_dev int8 [R][N]. */
+int piqmc_sa_replay(const piqmc_problem *pb, int R, int S, int M, const double *sched_host,
+                    int8_t *svec_dev, const uint16_t *perms_dev, uint32_t *rand_dev,
+                    uint64_t *draws_dev, uint32_t *ties_dev, void *stream);
+
+/* Host-memory drop-ins: same arguments and in-place semantics as the Cython
+ * functions, with the permutation stream pre-drawn by the caller from its
+ * RandomState and the libc rand() state passed explicitly.
+ *  qmc.QuantumAnneal: confs element (i,k) at confs[i*cs0 + k*cs1] (float64,
+ *  +-1); perms int64 [S*M][N]; rand_state32 in/out; *draws out. */
+int piqmc_quantum_anneal(const double *sched, int S, int mcsteps, int slices, float temp,
+                         int nspins, double *confs, int64_t cs0, int64_t cs1,
+                         const double *nbs, int maxnb, const int64_t *perms,
+                         uint32_t *rand_state32, uint64_t *draws, uint32_t *ties);
+/* sa.Anneal: svec element i at svec[i*ss]. */
+int piqmc_anneal(const double *sched, int S, int mcsteps, double *svec, int64_t ss, int nspins,
+                 const double *nbs, int maxnb, const int64_t *perms, uint32_t *rand_state32,
+                 uint64_t *draws, uint32_t *ties);
+
+/* ------------------------------------------------------------------ */
+/* Parallel sweep ("fixed" semantics: periodic Trotter neighbours,     */
+/* per-spin reset) -- graph-colored in space, parity-alternated in     */
+/* imaginary time, Philox4x32-10 draws.  The parallel counterpart of   */
+/* qmc.QuantumAnneal_parallel (piqmc/qmc.pyx:247-357).                 */
+/*  confs_dev int8 [R][P][N] in/out (P even).                          */
+/*  etrace_dev float64 [R][S] (or NULL): mean over slices of the       */
+/*  classical energy after each field step.                            */
+/*  block: threads per workgroup (0 = tuned default).                  */
+/* ------------------------------------------------------------------ */
+int piqmc_qa_colored(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
+                     const double *sched_host, int8_t *confs_dev, uint64_t seed,
+                     uint32_t replica_base, uint32_t sweep_base, double *etrace_dev,
+                     int block, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Energies.                                                           */
+/* ------------------------------------------------------------------ */
+/* Classical energy of each of B int8 configurations [B][N] from the
+ * neighbour table (pair entries weighted 1/2, field entries 1). */
+int piqmc_slice_energies(const piqmc_problem *pb, int B, const int8_t *spins_dev,
+                         double *E_dev, void *stream);
+/* sa.ClassicalIsingEnergy (piqmc/sa.pyx:25-44): E = -s.(J_off s) - sum(diag(J) s)
+ * for B float64 configurations [B][N] and a COO coupling list (host). */
+int piqmc_classical_energy(int nspins, int64_t nnz, const int64_t *rows, const int64_t *cols,
+                           const double *vals, int B, const double *spins, double *E);
+
+/* ------------------------------------------------------------------ */
+/* tools.GenerateNeighbors (piqmc/tools.pyx:28-96).                    */
+/*  (rows, cols, vals) list the DOK keys in insertion order.           */
+/*  order 0: CPython 2.7 dict iteration order (the reference's);       */
+/*  order 1: insertion order; order 2: row-major sorted.               */
+/*  nbs_out float64 [nspins][maxnb][2], zero-filled then written.      */
+/*  PIQMC_EINDEX when a spin has more than maxnb entries.              */
+/* ------------------------------------------------------------------ */
+int piqmc_generate_neighbors(int nspins, int maxnb, int64_t nnz, const int64_t *rows,
+                             const int64_t *cols, const double *vals, int order,
+                             double *nbs_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

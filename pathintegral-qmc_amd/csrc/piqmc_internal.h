@@ -1,0 +1,80 @@
+// Internal declarations shared by the HIP translation units of libpiqmc_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/piqmc_amd.h"
+#include "piqmc_fexp64.h"
+
+// ------------------------------------------------------------------ errors
+namespace piqmc {
+int set_error(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+}  // namespace piqmc
+
+#define PIQMC_HIP(call)                                                   \
+    do {                                                                  \
+        hipError_t e_ = (call);                                           \
+        if (e_ != hipSuccess) return piqmc::hip_fail(e_, #call);          \
+    } while (0)
+
+// ------------------------------------------------------------------ problem
+// A neighbour table (tools.GenerateNeighbors format) resident in HBM, with the
+// analysis the colored sweep needs.
+struct piqmc_problem {
+    int N = 0, Z = 0;
+    std::vector<int32_t> idx;  // [N][Z]  (int)nbs[i][z][0]
+    std::vector<float> J;      // [N][Z]  (float)nbs[i][z][1]
+    int2 *d_rows = nullptr;    // [N][Z]  (j, float bits of J), replay + generic path
+
+    // Graph coloring of the nonzero off-diagonal couplings.
+    int ncolors = 0;
+    std::vector<int> color;       // [N]
+    std::vector<int> color_off;   // [ncolors+1] offsets into order
+    int32_t *d_order = nullptr;   // [N] spins sorted by color
+
+    // +-1 coupling path: every nonzero entry is J=+-1 to another spin.
+    int pmj = 0;
+    int zmax = 0;                 // max nonzero off-diagonal entries per row
+    int32_t *d_cnb = nullptr;     // [zmax][N] neighbour of position q (color order)
+    uint32_t *d_cinfo = nullptr;  // [N] bits 0..7 negative-coupling mask, 8..11 z
+};
+
+// ------------------------------------------------------------------ device helpers
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4])
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+// Host J_perp, piqmc/qmc.pyx:95 (qmc.c:2055-2075).
+float host_jperp(double gamma, int P, float temp);
+
+// HIP events bracketing the main sweep kernel of the last call on this host
+// thread (read back by piqmc_last_kernel_ms).
+namespace piqmc {
+void timing_begin(hipStream_t st);
+void timing_end(hipStream_t st);
+}  // namespace piqmc

@@ -1,0 +1,367 @@
+// Replay sweeps: qmc.QuantumAnneal (piqmc/qmc.pyx:30-136) and sa.Anneal
+// (piqmc/sa.pyx:50-120) reproduced bit-for-bit on gfx950.
+//
+// A replica's sweep is a strictly sequential Markov chain (the reference keeps
+// a running ediff across the spins of a slice, D2, and draws libc rand() only
+// on the rejection branch), so one lane owns one replica and the replicas of
+// a workgroup advance in lockstep.  Per lane the kernel keeps, bit-packed in
+// LDS with lane-interleaved words (conflict-free):
+//   - the slice being swept (streamed from HBM once per slice),
+//   - slices 1 and P-1, the only Trotter neighbours the reference reads (D1),
+//   - the 31-word glibc TYPE_3 ring of its rand() stream.
+// The permutation stream is read from HBM as [sweep][position][replica] uint16,
+// one coalesced 2*L-byte load per attempt.
+//
+// Arithmetic is the reference's: ediff is float and every term is an exact
+// +-2*float product, so the float add equals the reference's double add
+// rounded to float; ediff/temp is IEEE float division; exp() is double; the
+// comparison is against rand()/2147483647.0 in double.
+#include <math.h>
+
+#include "piqmc_internal.h"
+
+namespace {
+
+struct ReplayArgs {
+    int R, N, P, Z, S, M, Wn, sa;
+    float temp;
+    const float *steps;     // [S] J_perp (QA) or float temperature (SA)
+    const int2 *rows;       // [N][Z] (j, J bits)
+    const uint16_t *perms;  // [S*M][N][R]
+    uint32_t *bits;         // [P][Wn][R]
+    uint32_t *rstate;       // [R][32]
+    unsigned long long *draws;
+    uint32_t *ties;
+};
+
+__device__ __forceinline__ uint32_t getbit(const uint32_t *reg, int L, int lane, int i)
+{
+    return (reg[(i >> 5) * L + lane] >> (i & 31)) & 1u;
+}
+
+__global__ void replay_kernel(ReplayArgs a)
+{
+    extern __shared__ uint32_t sm[];
+    const int L = blockDim.x, lane = threadIdx.x;
+    const int r = blockIdx.x * L + lane;
+    if (r >= a.R) return;  // no barriers below: each lane touches only its own words
+    const int N = a.N, P = a.P, Z = a.Z, Wn = a.Wn, R = a.R;
+    uint32_t *cur = sm, *s1, *sp, *ring;
+    if (a.sa) {
+        s1 = sp = cur;
+        ring = sm + Wn * L;
+    } else {
+        s1 = sm + Wn * L;
+        sp = (P == 2) ? s1 : sm + 2 * Wn * L;
+        ring = sm + 3 * Wn * L;
+    }
+    auto load = [&](uint32_t *reg, int k) {
+        for (int w = 0; w < Wn; ++w) reg[w * L + lane] = a.bits[((size_t)k * Wn + w) * R + r];
+    };
+    auto store = [&](const uint32_t *reg, int k) {
+        for (int w = 0; w < Wn; ++w) a.bits[((size_t)k * Wn + w) * R + r] = reg[w * L + lane];
+    };
+    if (a.sa) {
+        load(cur, 0);
+    } else {
+        load(s1, 1);
+        if (P != 2) load(sp, P - 1);
+    }
+    for (int w = 0; w < 31; ++w) ring[w * L + lane] = a.rstate[(size_t)r * 32 + w];
+    uint32_t f = a.rstate[(size_t)r * 32 + 31] % 31u;
+    unsigned long long nd = 0;
+    uint32_t nt = 0;
+    const float T = a.temp;
+    const int nslice = a.sa ? 1 : P;
+
+    for (int s = 0; s < a.S; ++s) {
+        const float sv = a.steps[s];
+        const float tdiv = a.sa ? sv : T;
+        for (int m = 0; m < a.M; ++m) {
+            const uint16_t *pg = a.perms + (size_t)(s * a.M + m) * N * R + r;
+            for (int k = 0; k < nslice; ++k) {
+                uint32_t *reg = a.sa ? cur : (k == 1 ? s1 : (k == P - 1 ? sp : cur));
+                const bool own = !a.sa && reg == cur;
+                if (own) load(cur, k);
+                float e = 0.0f;
+                int inext = pg[0];
+                for (int t = 0; t < N; ++t) {
+                    const int i = inext;
+                    if (t + 1 < N) inext = pg[(size_t)(t + 1) * R];
+                    const float m2 = getbit(reg, L, lane, i) ? 2.0f : -2.0f;  // -2*s_i
+                    if (a.sa) e = 0.0f;
+                    const int2 *row = a.rows + (size_t)i * Z;
+                    for (int z = 0; z < Z; ++z) {
+                        const int2 rw = row[z];
+                        const float J = __int_as_float(rw.y);
+                        if (rw.x == i) {
+                            e = e + m2 * J;
+                        } else {
+                            const float sj = getbit(reg, L, lane, rw.x) ? -1.0f : 1.0f;
+                            e = e + m2 * (J * sj);
+                        }
+                    }
+                    bool flip;
+                    if (!a.sa) {
+                        const float sl = getbit(sp, L, lane, i) ? -1.0f : 1.0f;  // slice P-1
+                        e = e + m2 * (sv * sl);
+                        const float sr = getbit(s1, L, lane, i) ? -1.0f : 1.0f;  // slice 1
+                        e = e + m2 * (sv * sr);
+                        flip = e > 0.0f;
+                    } else {
+                        flip = e >= 0.0f;
+                    }
+                    if (!flip) {
+                        const uint32_t rp = f >= 3 ? f - 3 : f + 28;
+                        const uint32_t v = ring[f * L + lane] + ring[rp * L + lane];
+                        ring[f * L + lane] = v;
+                        f = f == 30 ? 0 : f + 1;
+                        ++nd;
+                        const double u = (double)(v >> 1) / 2147483647.0;
+                        const float q = e / tdiv;
+                        const double ex = exp((double)q);
+                        flip = ex > u;
+                        if (ex > 0.0 && fabs(ex - u) <= ex * 0x1p-50) ++nt;
+                    }
+                    if (flip) reg[(i >> 5) * L + lane] ^= 1u << (i & 31);
+                }
+                if (own) store(cur, k);
+            }
+        }
+    }
+    if (a.sa) {
+        store(cur, 0);
+    } else {
+        store(s1, 1);
+        if (P != 2) store(sp, P - 1);
+    }
+    for (int w = 0; w < 31; ++w) a.rstate[(size_t)r * 32 + w] = ring[w * L + lane];
+    a.rstate[(size_t)r * 32 + 31] = f;
+    a.draws[r] = nd;
+    a.ties[r] = nt;
+}
+
+// int8 [R][P][N] -> bits [P][Wn][R] (bit set = spin -1)
+__global__ void pack_lanes_kernel(const int8_t *confs, uint32_t *bits, int R, int P, int N, int Wn)
+{
+    const size_t tot = (size_t)R * P * Wn;
+    for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < tot;
+         x += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(x % R);
+        const size_t kw = x / R;
+        const int w = (int)(kw % Wn), k = (int)(kw / Wn);
+        const int8_t *src = confs + ((size_t)r * P + k) * N;
+        uint32_t word = 0;
+        for (int b = 0; b < 32; ++b) {
+            const int i = w * 32 + b;
+            if (i < N && src[i] < 0) word |= 1u << b;
+        }
+        bits[x] = word;
+    }
+}
+
+__global__ void unpack_lanes_kernel(const uint32_t *bits, int8_t *confs, int R, int P, int N, int Wn)
+{
+    const size_t tot = (size_t)R * P * N;
+    for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < tot;
+         x += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(x % N);
+        const size_t rk = x / N;
+        const int k = (int)(rk % P), r = (int)(rk / P);
+        const uint32_t w = bits[((size_t)k * Wn + (i >> 5)) * R + r];
+        confs[x] = ((w >> (i & 31)) & 1u) ? (int8_t)-1 : (int8_t)1;
+    }
+}
+
+// Flags any permutation entry outside [0, N) (the sweep indexes LDS and the
+// neighbour table with it).
+__global__ void check_perms_kernel(const uint16_t *perms, size_t n, int N, int *bad)
+{
+    for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < n;
+         x += (size_t)gridDim.x * blockDim.x)
+        if ((int)perms[x] >= N) atomicOr(bad, 1);
+}
+
+int grid_for(size_t n, int block)
+{
+    size_t g = (n + block - 1) / block;
+    return (int)(g > 65535 ? 65535 : (g < 1 ? 1 : g));
+}
+
+int run_replay(const piqmc_problem *pb, int R, int P, int S, int M, float temp, int sa,
+               const float *steps_host, int8_t *confs, const uint16_t *perms, uint32_t *rand,
+               uint64_t *draws, uint32_t *ties, hipStream_t st)
+{
+    const int N = pb->N, Wn = (N + 31) / 32;
+    const int nreg = sa ? 1 : (P == 2 ? 2 : 3);
+    // Lanes per workgroup: the widest power of two whose LDS image fits.
+    int L = 64;
+    auto lds = [&](int l) { return (size_t)(nreg * Wn + 31) * l * 4; };
+    while (L > 1 && (lds(L) > 160 * 1024 || L / 2 >= R)) L /= 2;
+    if (lds(L) > 160 * 1024)
+        return piqmc::set_error(PIQMC_EINVAL, "replay: nspins too large for one workgroup's LDS");
+    const size_t nperm = (size_t)S * M * N * R;
+    if (nperm && N < 65536) {
+        int *d_bad = nullptr, bad = 0;
+        PIQMC_HIP(hipMallocAsync((void **)&d_bad, sizeof(int), st));
+        PIQMC_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), st));
+        check_perms_kernel<<<grid_for(nperm, 256), 256, 0, st>>>(perms, nperm, N, d_bad);
+        PIQMC_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st));
+        PIQMC_HIP(hipFreeAsync(d_bad, st));
+        PIQMC_HIP(hipStreamSynchronize(st));
+        if (bad) return piqmc::set_error(PIQMC_EINVAL, "permutation entry out of range [0, nspins)");
+    }
+    float *d_steps = nullptr;
+    uint32_t *d_bits = nullptr;
+    unsigned long long *d_draws = reinterpret_cast<unsigned long long *>(draws);
+    PIQMC_HIP(hipMallocAsync((void **)&d_steps, sizeof(float) * (S > 0 ? S : 1), st));
+    PIQMC_HIP(hipMallocAsync((void **)&d_bits, sizeof(uint32_t) * (size_t)P * Wn * R, st));
+    if (S > 0) PIQMC_HIP(hipMemcpyAsync(d_steps, steps_host, sizeof(float) * S, hipMemcpyHostToDevice, st));
+    pack_lanes_kernel<<<grid_for((size_t)R * P * Wn, 256), 256, 0, st>>>(confs, d_bits, R, P, N, Wn);
+    ReplayArgs a;
+    a.R = R; a.N = N; a.P = P; a.Z = pb->Z; a.S = S; a.M = M; a.Wn = Wn; a.sa = sa;
+    a.temp = temp;
+    a.steps = d_steps;
+    a.rows = pb->d_rows;
+    a.perms = perms;
+    a.bits = d_bits;
+    a.rstate = rand;
+    a.draws = d_draws;
+    a.ties = ties;
+    (void)hipFuncSetAttribute((const void *)replay_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds(L));
+    piqmc::timing_begin(st);
+    replay_kernel<<<(R + L - 1) / L, L, lds(L), st>>>(a);
+    piqmc::timing_end(st);
+    PIQMC_HIP(hipGetLastError());
+    unpack_lanes_kernel<<<grid_for((size_t)R * P * N, 256), 256, 0, st>>>(d_bits, confs, R, P, N, Wn);
+    PIQMC_HIP(hipFreeAsync(d_bits, st));
+    PIQMC_HIP(hipFreeAsync(d_steps, st));
+    PIQMC_HIP(hipGetLastError());
+    return PIQMC_OK;
+}
+
+}  // namespace
+
+extern "C" int piqmc_qa_replay(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
+                               const double *sched, int8_t *confs, const uint16_t *perms,
+                               uint32_t *rand, uint64_t *draws, uint32_t *ties, void *stream)
+{
+    if (!pb || R < 1 || S < 0 || M < 0 || (S > 0 && !sched) || !confs || !rand || !draws || !ties)
+        return piqmc::set_error(PIQMC_EINVAL, "qa_replay: bad arguments");
+    if (P < 2) return piqmc::set_error(PIQMC_EINVAL, "QuantumAnneal needs slices >= 2 (reads slice 1)");
+    if (pb->N > 65536) return piqmc::set_error(PIQMC_EINVAL, "replay supports nspins <= 65536");
+    if ((long)S * M > 0 && !perms) return piqmc::set_error(PIQMC_EINVAL, "qa_replay: null perms");
+    std::vector<float> jp(S > 0 ? S : 0);
+    for (int s = 0; s < S; ++s) {
+        if ((float)P * temp == 0.0f) return piqmc::set_error(PIQMC_EZERODIV, "float division");
+        jp[s] = host_jperp(sched[s], P, temp);
+    }
+    if ((long)S * M > 0 && temp == 0.0f) return piqmc::set_error(PIQMC_EZERODIV, "float division");
+    return run_replay(pb, R, P, S, M, temp, 0, jp.data(), confs, perms, rand, draws, ties,
+                      (hipStream_t)stream);
+}
+
+extern "C" int piqmc_sa_replay(const piqmc_problem *pb, int R, int S, int M, const double *sched,
+                               int8_t *svec, const uint16_t *perms, uint32_t *rand, uint64_t *draws,
+                               uint32_t *ties, void *stream)
+{
+    if (!pb || R < 1 || S < 0 || M < 0 || (S > 0 && !sched) || !svec || !rand || !draws || !ties)
+        return piqmc::set_error(PIQMC_EINVAL, "sa_replay: bad arguments");
+    if (pb->N > 65536) return piqmc::set_error(PIQMC_EINVAL, "replay supports nspins <= 65536");
+    if ((long)S * M > 0 && !perms) return piqmc::set_error(PIQMC_EINVAL, "sa_replay: null perms");
+    std::vector<float> tv(S > 0 ? S : 0);
+    for (int s = 0; s < S; ++s) tv[s] = (float)sched[s];  // piqmc/sa.pyx:96
+    return run_replay(pb, R, 1, S, M, 0.0f, 1, tv.data(), svec, perms, rand, draws, ties,
+                      (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------ host drop-ins
+namespace {
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t n) { return hipMalloc((void **)&p, sizeof(T) * (n ? n : 1)); }
+};
+
+int dropin(const double *sched, int S, int M, int P, float temp, int N, double *confs, int64_t cs0,
+           int64_t cs1, const double *nbs, int Z, const int64_t *perms, uint32_t *rs,
+           uint64_t *draws_out, uint32_t *ties_out, int sa)
+{
+    if (N < 1 || Z < 1 || S < 0 || M < 0 || !confs || !nbs || !rs)
+        return piqmc::set_error(PIQMC_EINVAL, "bad arguments");
+    if (!sa && P < 2) return piqmc::set_error(PIQMC_EINVAL, "QuantumAnneal needs slices >= 2 (reads slice 1)");
+    if (rs[31] > 30) return piqmc::set_error(PIQMC_EINVAL, "bad TYPE_3 state");
+    const size_t G = (size_t)S * M;
+    std::vector<int8_t> c8((size_t)P * N);
+    for (int k = 0; k < P; ++k)
+        for (int i = 0; i < N; ++i) {
+            const double v = confs[i * cs0 + k * cs1];
+            if (v != 1.0 && v != -1.0)
+                return piqmc::set_error(PIQMC_EINVAL, "spin configurations must hold +1/-1 values");
+            c8[(size_t)k * N + i] = v < 0 ? -1 : 1;
+        }
+    std::vector<uint16_t> p16(G * N);
+    for (size_t g = 0; g < G; ++g)
+        for (int t = 0; t < N; ++t) {
+            const int64_t v = perms[g * N + t];
+            if (v < 0 || v >= N) return piqmc::set_error(PIQMC_EINVAL, "permutation entry out of range");
+            p16[g * N + t] = (uint16_t)v;
+        }
+    piqmc_problem *pb = nullptr;
+    int rc = piqmc_problem_create(N, Z, nbs, &pb);
+    if (rc) return rc;
+    struct Guard {
+        piqmc_problem *p;
+        ~Guard() { piqmc_problem_destroy(p); }
+    } guard{pb};
+    DevBuf<int8_t> dc;
+    DevBuf<uint16_t> dp;
+    DevBuf<uint32_t> dr, dt;
+    DevBuf<uint64_t> dd;
+    PIQMC_HIP(dc.alloc(c8.size()));
+    PIQMC_HIP(dp.alloc(p16.size()));
+    PIQMC_HIP(dr.alloc(32));
+    PIQMC_HIP(dt.alloc(1));
+    PIQMC_HIP(dd.alloc(1));
+    PIQMC_HIP(hipMemcpy(dc.p, c8.data(), c8.size(), hipMemcpyHostToDevice));
+    if (!p16.empty()) PIQMC_HIP(hipMemcpy(dp.p, p16.data(), p16.size() * 2, hipMemcpyHostToDevice));
+    PIQMC_HIP(hipMemcpy(dr.p, rs, 32 * 4, hipMemcpyHostToDevice));
+    rc = sa ? piqmc_sa_replay(pb, 1, S, M, sched, dc.p, dp.p, dr.p, dd.p, dt.p, nullptr)
+            : piqmc_qa_replay(pb, 1, P, S, M, temp, sched, dc.p, dp.p, dr.p, dd.p, dt.p, nullptr);
+    if (rc) return rc;
+    PIQMC_HIP(hipDeviceSynchronize());
+    uint64_t nd = 0;
+    uint32_t nt = 0;
+    PIQMC_HIP(hipMemcpy(c8.data(), dc.p, c8.size(), hipMemcpyDeviceToHost));
+    PIQMC_HIP(hipMemcpy(rs, dr.p, 32 * 4, hipMemcpyDeviceToHost));
+    PIQMC_HIP(hipMemcpy(&nd, dd.p, 8, hipMemcpyDeviceToHost));
+    PIQMC_HIP(hipMemcpy(&nt, dt.p, 4, hipMemcpyDeviceToHost));
+    for (int k = 0; k < P; ++k)
+        for (int i = 0; i < N; ++i) confs[i * cs0 + k * cs1] = (double)c8[(size_t)k * N + i];
+    if (draws_out) *draws_out = nd;
+    if (ties_out) *ties_out = nt;
+    return PIQMC_OK;
+}
+}  // namespace
+
+extern "C" int piqmc_quantum_anneal(const double *sched, int S, int mcsteps, int slices, float temp,
+                                    int nspins, double *confs, int64_t cs0, int64_t cs1,
+                                    const double *nbs, int maxnb, const int64_t *perms,
+                                    uint32_t *rs, uint64_t *draws, uint32_t *ties)
+{
+    return dropin(sched, S, mcsteps, slices, temp, nspins, confs, cs0, cs1, nbs, maxnb, perms, rs,
+                  draws, ties, 0);
+}
+
+extern "C" int piqmc_anneal(const double *sched, int S, int mcsteps, double *svec, int64_t ss,
+                            int nspins, const double *nbs, int maxnb, const int64_t *perms,
+                            uint32_t *rs, uint64_t *draws, uint32_t *ties)
+{
+    return dropin(sched, S, mcsteps, 1, 0.0f, nspins, svec, ss, 0, nbs, maxnb, perms, rs, draws,
+                  ties, 1);
+}

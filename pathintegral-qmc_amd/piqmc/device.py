@@ -1,0 +1,169 @@
+"""Batched device API: many replicas resident in HBM, one launch per call.
+
+A :class:`Problem` holds a neighbour table (tools.GenerateNeighbors format) in
+HBM together with its graph coloring.  Its methods take torch tensors that
+already live on the GPU (torch is used only for device memory and streams):
+
+* ``qa_replay``   -- qmc.QuantumAnneal for R replicas, bit-exact replay
+                     (piqmc/qmc.pyx:30-136);
+* ``sa_replay``   -- sa.Anneal for R replicas (piqmc/sa.pyx:50-120);
+* ``qa_colored``  -- the graph-colored, parity-alternated parallel sweep;
+* ``slice_energies`` -- classical energy of every slice.
+
+Spin tensors are int8 +1/-1: replicas x slices x spins ([R, P, N]).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+def _stream(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _check_dev(t, dtype, shape, name):
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError("%s must be a CUDA (HIP) torch tensor" % name)
+    if t.dtype != dtype:
+        raise ValueError("%s must have dtype %s, got %s" % (name, dtype, t.dtype))
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError("%s must have shape %s, got %s" % (name, tuple(shape), tuple(t.shape)))
+    if not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % name)
+
+
+def last_kernel_ms():
+    """Device milliseconds of the main sweep kernel of the last replay /
+    parallel call from this thread (HIP events on that call's stream)."""
+    ms = C.c_float()
+    _lib.call("piqmc_last_kernel_ms", C.byref(ms))
+    return ms.value
+
+
+def rand_seed(seed):
+    """glibc srand(seed) image of the TYPE_3 rand() state, uint32[32]."""
+    st = np.zeros(32, np.uint32)
+    _lib.call("piqmc_rand_seed", int(seed) & 0xFFFFFFFF, st.ctypes.data)
+    return st
+
+
+def rand_states(seeds):
+    return np.stack([rand_seed(s) for s in seeds])
+
+
+def perm_streams(seeds, nspins, nsweeps, nthreads=0):
+    """uint16 [nsweeps, nspins, R]: stream r is what QuantumAnneal draws from
+    numpy RandomState(seeds[r]) (first permutation of range(N), then chained)."""
+    seeds = np.ascontiguousarray(seeds, np.uint32)
+    out = np.empty((nsweeps, nspins, len(seeds)), np.uint16)
+    _lib.call("piqmc_perm_streams", seeds.ctypes.data, len(seeds), int(nspins), int(nsweeps),
+              out.ctypes.data, int(nthreads))
+    return out
+
+
+class Problem:
+    """A neighbour table resident in HBM (piqmc_problem handle)."""
+
+    def __init__(self, nbs):
+        nbs = np.ascontiguousarray(nbs, np.float64)
+        if nbs.ndim != 3 or nbs.shape[2] != 2:
+            raise ValueError("nbs must be [nspins, maxnb, 2]")
+        self.nspins, self.maxnb = nbs.shape[0], nbs.shape[1]
+        h = C.c_void_p()
+        _lib.call("piqmc_problem_create", self.nspins, self.maxnb, nbs.ctypes.data, C.byref(h))
+        self._h = h
+        vals = [C.c_int() for _ in range(5)]
+        _lib.call("piqmc_problem_info", self._h, *[C.byref(v) for v in vals])
+        self.ncolors, self.pm_j, self.zmax = vals[2].value, bool(vals[3].value), vals[4].value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib.piqmc_problem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def coloring(self):
+        """The spatial coloring in use, int32 [nspins]."""
+        out = np.zeros(self.nspins, np.int32)
+        _lib.call("piqmc_problem_get_coloring", self._h, out.ctypes.data)
+        return out
+
+    def set_coloring(self, color):
+        color = np.ascontiguousarray(color, np.int32)
+        nc = int(color.max()) + 1
+        _lib.call("piqmc_problem_set_coloring", self._h, nc, color.ctypes.data)
+        self.ncolors = nc
+
+    # ------------------------------------------------------------ replay
+    def qa_replay(self, confs, sched, mcsteps, temp, perms, rand_state, stream=None):
+        """confs int8 [R,P,N] in place; perms int16/uint16 [S*M, N, R];
+        rand_state int32 [R, 32] in place.  Returns (draws int64 [R], ties int32 [R])."""
+        import torch
+        R, P, N = confs.shape
+        sched = np.ascontiguousarray(sched, np.float64)
+        S, M = len(sched), int(mcsteps)
+        _check_dev(confs, torch.int8, (R, P, self.nspins), "confs")
+        if perms.dtype not in (torch.int16, torch.uint16):
+            raise ValueError("perms must be 16-bit")
+        _check_dev(perms, perms.dtype, (S * M, N, R), "perms")
+        _check_dev(rand_state, torch.int32, (R, 32), "rand_state")
+        draws = torch.zeros(R, dtype=torch.int64, device=confs.device)
+        ties = torch.zeros(R, dtype=torch.int32, device=confs.device)
+        _lib.call("piqmc_qa_replay", self._h, R, P, S, M, float(temp), sched.ctypes.data,
+                  confs.data_ptr(), perms.data_ptr(), rand_state.data_ptr(), draws.data_ptr(),
+                  ties.data_ptr(), _stream(stream))
+        return draws, ties
+
+    def sa_replay(self, svec, sched, mcsteps, perms, rand_state, stream=None):
+        """svec int8 [R,N] in place; perms [S*M, N, R]; rand_state int32 [R,32]."""
+        import torch
+        R, N = svec.shape
+        sched = np.ascontiguousarray(sched, np.float64)
+        S, M = len(sched), int(mcsteps)
+        _check_dev(svec, torch.int8, (R, self.nspins), "svec")
+        _check_dev(perms, perms.dtype, (S * M, N, R), "perms")
+        _check_dev(rand_state, torch.int32, (R, 32), "rand_state")
+        draws = torch.zeros(R, dtype=torch.int64, device=svec.device)
+        ties = torch.zeros(R, dtype=torch.int32, device=svec.device)
+        _lib.call("piqmc_sa_replay", self._h, R, S, M, sched.ctypes.data, svec.data_ptr(),
+                  perms.data_ptr(), rand_state.data_ptr(), draws.data_ptr(), ties.data_ptr(),
+                  _stream(stream))
+        return draws, ties
+
+    # ------------------------------------------------------------ parallel
+    def qa_colored(self, confs, sched, mcsteps, temp, seed, replica_base=0, sweep_base=0,
+                   trace=False, block=0, stream=None):
+        """Parallel sweep of confs int8 [R,P,N] in place.  Returns the
+        per-field-step mean slice energy float64 [R,S] when trace=True."""
+        import torch
+        R, P, N = confs.shape
+        sched = np.ascontiguousarray(sched, np.float64)
+        S = len(sched)
+        _check_dev(confs, torch.int8, (R, P, self.nspins), "confs")
+        et = torch.zeros((R, S), dtype=torch.float64, device=confs.device) if trace else None
+        _lib.call("piqmc_qa_colored", self._h, R, P, S, int(mcsteps), float(temp),
+                  sched.ctypes.data, confs.data_ptr(), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                  int(replica_base), int(sweep_base), et.data_ptr() if trace else None,
+                  int(block), _stream(stream))
+        return et
+
+    # ------------------------------------------------------------ energies
+    def slice_energies(self, spins, stream=None):
+        """Classical energy of each row of int8 spins [..., N] (float64)."""
+        import torch
+        flat = spins.reshape(-1, self.nspins)
+        _check_dev(flat, torch.int8, flat.shape, "spins")
+        E = torch.zeros(flat.shape[0], dtype=torch.float64, device=spins.device)
+        _lib.call("piqmc_slice_energies", self._h, flat.shape[0], flat.data_ptr(), E.data_ptr(),
+                  _stream(stream))
+        return E.reshape(spins.shape[:-1])

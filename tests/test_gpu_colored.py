@@ -1,0 +1,111 @@
+"""Parallel (graph-colored, parity-alternated) sweep vs its CPU twin.
+
+The device result must equal orc_qa_colored bit-for-bit (same phases, same
+Philox4x32-10 counters, same fixed-semantics arithmetic) on both kernel paths
+(bit-sliced +-1 path and per-bit general path), for every workgroup size.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import pm_j_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(gpu, nbs, P, sched, M, T, R, seed, rbase=0, sbase=0, block=0, coloring=None):
+    import torch
+    from piqmc import device
+    N = nbs.shape[0]
+    pb = device.Problem(nbs)
+    if coloring is not None:
+        pb.set_coloring(coloring)
+    color = pb.coloring()
+    rng = np.random.RandomState(seed)
+    c0 = (2 * rng.randint(2, size=(R, P, N)) - 1).astype(np.int8)
+    c8 = torch.tensor(c0, device=gpu)
+    et = pb.qa_colored(c8, sched, M, T, seed, replica_base=rbase, sweep_base=sbase, trace=True,
+                       block=block)
+    torch.cuda.synchronize()
+    got, etg = c8.cpu().numpy(), et.cpu().numpy()
+    ref, etr = [], []
+    for r in range(R):
+        sp = c0[r].copy()
+        etr.append(O.qa_colored(sched, M, P, T, sp, nbs, color, int(color.max()) + 1, seed,
+                                replica=rbase + r, sweep_base=sbase, trace=True))
+        ref.append(sp)
+    return pb, got, np.array(ref), etg, np.array(etr)
+
+
+@pytest.mark.parametrize("L,P,S,T", [(8, 20, 100, 0.05), (8, 64, 40, 1 / 64), (6, 128, 10, 0.02),
+                                     (4, 2, 9, 0.5), (10, 16, 30, 0.05)])
+def test_colored_pm_j_bit_exact(gpu, L, P, S, T):
+    N, nbs, _, _, _ = pm_j_problem(L, seed=L + P)
+    sched = np.linspace(3.0, 1e-8, S)
+    pb, got, ref, etg, etr = _run(gpu, nbs, P, sched, 1, T, R=5, seed=1234 + P, rbase=7, sbase=3)
+    assert pb.pm_j and pb.ncolors == 2
+    assert np.array_equal(got, ref)
+    assert np.array_equal(etg, etr)          # +-1 energies are exact
+
+
+def test_colored_3d_pm_j(gpu):
+    N, nbs, _, _, _ = pm_j_problem(4, dim=3, seed=9)
+    pb, got, ref, etg, etr = _run(gpu, nbs, 12, np.linspace(3.0, 1e-8, 20), 2, 0.1, R=3, seed=5)
+    assert pb.pm_j and pb.zmax == 6
+    assert np.array_equal(got, ref) and np.array_equal(etg, etr)
+
+
+def test_colored_general_path_fields(gpu):
+    # 3-D +-J with Gaussian fields (the large sparse workload's structure)
+    N, nbs, _, _, _ = pm_j_problem(4, dim=3, seed=4, field_sigma=0.1)
+    pb, got, ref, etg, etr = _run(gpu, nbs, 16, np.linspace(3.0, 1e-8, 15), 1, 0.05, R=3, seed=8)
+    assert not pb.pm_j
+    assert np.array_equal(got, ref)
+    np.testing.assert_allclose(etg, etr, rtol=1e-12, atol=1e-12)
+
+
+def test_colored_general_path_frustrated(gpu):
+    # odd ring + chords: non-bipartite, greedy coloring with >2 colors, float J
+    from piqmc import instances
+    rng = np.random.RandomState(3)
+    N = 21
+    rows = np.concatenate([np.arange(N), np.arange(0, N, 3)])
+    cols = np.concatenate([(np.arange(N) + 1) % N, (np.arange(0, N, 3) + 7) % N])
+    lo, hi = np.minimum(rows, cols), np.maximum(rows, cols)
+    vals = rng.uniform(-2, 2, lo.size)
+    nbs = instances.neighbors(N, lo, hi, vals)
+    pb, got, ref, etg, etr = _run(gpu, nbs, 8, np.linspace(2.0, 1e-8, 25), 1, 0.1, R=4, seed=21)
+    assert pb.ncolors >= 3
+    assert np.array_equal(got, ref)
+    np.testing.assert_allclose(etg, etr, rtol=1e-12, atol=1e-12)
+
+
+def test_colored_block_size_invariance(gpu):
+    # every workgroup size takes identical decisions (pinning any tuned tile)
+    N, nbs, _, _, _ = pm_j_problem(16, seed=3)
+    outs = []
+    for block in (64, 128, 256, 512, 1024):
+        _, got, ref, _, _ = _run(gpu, nbs, 32, np.linspace(3.0, 1e-8, 12), 1, 1 / 32, R=3, seed=77,
+                                 block=block)
+        assert np.array_equal(got, ref)
+        outs.append(got)
+    assert all(np.array_equal(outs[0], o) for o in outs)
+
+
+def test_colored_full_size_properties(gpu):
+    # 1024-spin glass, P=64, T=1/64, 500 steps, 256 replicas: per-replica energy
+    # trace consistent with a final slice-energy evaluation, spins stay +-1,
+    # and the anneal lowers the energy to near the ground state (< -1.3/spin).
+    import torch
+    from piqmc import device
+    N, nbs, _, _, _ = pm_j_problem(32, seed=2024)
+    pb = device.Problem(nbs)
+    R, P = 256, 64
+    rng = np.random.RandomState(0)
+    c8 = torch.tensor((2 * rng.randint(2, size=(R, P, N)) - 1).astype(np.int8), device=gpu)
+    et = pb.qa_colored(c8, np.linspace(3.0, 1e-8, 500), 1, 1 / 64, 99, trace=True)
+    E = pb.slice_energies(c8)
+    torch.cuda.synchronize()
+    assert torch.all((c8 == 1) | (c8 == -1))
+    np.testing.assert_allclose(et[:, -1].cpu().numpy(), E.mean(1).cpu().numpy(), rtol=0, atol=1e-9)
+    assert float(E.min(1).values.mean()) / N < -1.3

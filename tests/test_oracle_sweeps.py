@@ -1,0 +1,199 @@
+"""Pins of the oracle's sweep restatements (CPU).
+
+Compat sweeps (qmc.QuantumAnneal / sa.Anneal with the reference's defects) are
+pinned by hand-worked traces whose expected states were derived step by step
+from piqmc/qmc.pyx:93-136 and piqmc/sa.pyx:93-120 (worked in the comments), by
+the reference's own outcome tests (testing/test_boixo.py), and by known
+ground-state energies.  Fixed-semantics sweeps (sequential and colored) are
+pinned by exact enumeration of the Boltzmann law exp(-A/T) of the
+(d+1)-dimensional action they must leave invariant.
+"""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import boixo_J
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ------------------------------------------------------------ hand traces
+def test_trace_running_ediff_D2():
+    # N=2, J01=+1, P=2, J_perp=0 (Gamma=1e6 -> tanh=1 -> log=0), T=1e-3.
+    # slice 0 = [+1,-1], slice 1 = [+1,+1]; permutation [0,1]; srand(1).
+    # slice 0: spin0 e=-2*(+1)*(1*-1)=+2 >0 -> flip; spin1 e=2+(-2*-1)*(1*-1)=0
+    #          (running sum, D2) -> rand #1 = 0.84 < exp(0)=1 -> flip.
+    # slice 1: spin0 e=-2 -> rand #2, exp(-2000)=0 -> stay;
+    #          spin1 e=-2-2=-4 -> rand #3 -> stay.
+    nbs = np.array([[[1, 1.0]], [[0, 1.0]]])
+    confs = np.array([[1.0, 1.0], [-1.0, 1.0]])
+    d = O.qa_compat(np.array([1e6]), 1, 2, 1e-3, 2, confs, nbs, np.array([[0, 1]]), O.Type3(1))
+    assert confs.tolist() == [[-1.0, 1.0], [1.0, 1.0]]
+    assert d == 3
+
+
+def test_trace_fixed_trotter_slices_D1():
+    # N=1 (zero field row), P=4, T=0.25, Gamma=0.01: J_perp=-0.5*log(tanh(0.01))~2.30.
+    # D1: every slice couples to slices P-1=3 and 1.  Start [+1,-1,+1,-1].
+    # k=0: e=+2Jp+2Jp>0 -> -1.   k=1: neighbours 3 (-1) and itself (-1):
+    # e=-4Jp -> rand #1 (0.84) vs exp(-36.8) -> stay.  k=2: e=+4Jp -> -1.
+    # k=3: itself and slice 1: e=-4Jp -> rand #2 -> stay.  => all -1, 2 draws.
+    # (With periodic k-1,k+1 neighbours slice 1 would see e=0 and flip.)
+    nbs = np.array([[[0, 0.0]]])
+    confs = np.array([[1.0, -1.0, 1.0, -1.0]])
+    d = O.qa_compat(np.array([0.01]), 1, 4, 0.25, 1, confs, nbs, np.array([[0]]), O.Type3(1))
+    assert confs.tolist() == [[-1.0, -1.0, -1.0, -1.0]]
+    assert d == 2
+
+
+def test_trace_field_term():
+    # N=1 field h=0.5 (self row): e=-2*s*h = +1 for s=-1 -> flip in both slices.
+    nbs = np.array([[[0, 0.5]]])
+    confs = np.array([[-1.0, -1.0]])
+    d = O.qa_compat(np.array([1e6]), 1, 2, 1.0, 1, confs, nbs, np.array([[0]]), O.Type3(1))
+    assert confs.tolist() == [[1.0, 1.0]] and d == 0
+
+
+def test_trace_sa_per_spin_reset_and_ge():
+    # Chain 0-1-2, J=+1, rows padded with [0,0]; svec [+1,+1,-1]; perm [1,0,2].
+    # spin1: e=-2*(1)+(-2)*(-1)=0 -> ">=" accepts without a draw -> [+1,-1,-1]
+    # spin0: e=+2 -> flip -> [-1,-1,-1];  spin2: e=-2 -> rand #1 -> stay.
+    nbs = np.array([[[1, 1.0], [0, 0.0]], [[0, 1.0], [2, 1.0]], [[1, 1.0], [0, 0.0]]])
+    sv = np.array([1.0, 1.0, -1.0])
+    d = O.sa_compat(np.array([1e-3]), 1, sv, nbs, np.array([[1, 0, 2]]), O.Type3(1))
+    assert sv.tolist() == [-1.0, -1.0, -1.0] and d == 1
+
+
+def test_compat_strided_confs_same_as_contiguous():
+    rng = np.random.RandomState(3)
+    N, P = 6, 5
+    nbs = np.zeros((N, 2, 2))
+    for i in range(N):
+        nbs[i, 0] = [(i + 1) % N, rng.choice([-1.0, 1.0])]
+        nbs[i, 1] = [(i - 1) % N, nbs[(i - 1) % N, 0, 1]]
+    sv = 2.0 * rng.randint(2, size=N) - 1
+    cf = np.tile(sv, (P, 1)).T                      # F-ordered view
+    cc = np.ascontiguousarray(cf)
+    perms = O.MT(9).perm_stream(N, 4)
+    sched = np.linspace(2.0, 0.1, 4)
+    d1 = O.qa_compat(sched, 1, P, 0.2, N, cf, nbs, perms, O.Type3(4))
+    d2 = O.qa_compat(sched, 1, P, 0.2, N, cc, nbs, perms, O.Type3(4))
+    assert np.array_equal(cf, cc) and d1 == d2
+
+
+# ------------------------------------------------------------ reference outcome tests
+def _boixo_nbs():
+    from neighbors_py27 import generate_neighbors
+    J = boixo_J()
+    keys = list(J.keys())
+    return generate_neighbors(8, keys, [J[k] for k in keys], 4)
+
+
+def _boixo_energy(s):
+    J = boixo_J().tocoo()
+    return O.energy_coo(s, J.row, J.col, J.data)
+
+
+def test_classical_energy_known_answers():
+    # testing/test_boixo.py:60-70
+    svecs = [[-1, 1, -1, 1, 1, -1, -1, 1], [1, 1, 1, 1, 1, -1, 1, -1],
+             [1, -1, 1, 1, -1, -1, -1, -1], [1, -1, -1, 1, 1, -1, 1, 1]]
+    for v, e in zip(svecs, [4.0, -8.0, -4.0, 0.0]):
+        assert _boixo_energy(np.array(v, float)) == e
+        assert O.energy_nbs(np.array(v, np.int8), _boixo_nbs())[0] == e
+
+
+def test_boixo_sa_outcome():
+    # testing/test_boixo.py:72-88 with libc rand() from srand(1)
+    nbs, rng, rs = _boixo_nbs(), np.random.RandomState(123), O.Type3(1)
+    for _ in range(5):
+        sv = np.array([2 * rng.randint(2) - 1 for _ in range(8)], dtype=np.float64)
+        sched = np.linspace(1.0, 0.01, 10)
+        perms = [rng.permutation(range(8))]
+        for _ in range(30):
+            perms.append(rng.permutation(perms[-1]))
+        O.sa_compat(sched, 3, sv, nbs, np.array(perms[:30]), rs)
+        assert np.sum(sv[:4]) == 4 or np.sum(sv) == -8
+        assert _boixo_energy(sv) == -8.0
+
+
+def test_boixo_qmc_outcome():
+    # testing/test_boixo.py:90-114 with libc rand() from srand(1)
+    nbs, rng = _boixo_nbs(), np.random.RandomState(123)
+    confs = np.tile(np.array([2 * rng.randint(2) - 1 for _ in range(8)], dtype=np.float64),
+                    (5, 1)).T
+    sched = np.linspace(0.5, 1e-8, 10)
+    perms = [rng.permutation(range(8))]
+    for _ in range(30):
+        perms.append(rng.permutation(perms[-1]))
+    O.qa_compat(sched, 3, 5, 0.01, 8, confs, nbs, np.array(perms[:30]), O.Type3(1))
+    E = [_boixo_energy(confs[:, k]) for k in range(5)]
+    assert np.sum(E) == -8.0 * 5
+    for k in range(5):
+        assert np.sum(confs[:4, k]) == 4.0 or np.sum(confs[:, k]) == -8.0
+
+
+def test_instance_ground_state_energy():
+    # examples/spinglass32.py:33,77 -- exact GS of the 32x32 instance
+    a = np.loadtxt(os.path.join(GOLDEN, "inst_0_32x32.txt"))
+    ups = np.loadtxt(os.path.join(GOLDEN, "spinglass32_gs_up.txt"), dtype=int) - 1
+    gs = -np.ones(1024)
+    gs[ups] = 1
+    E = O.energy_coo(gs, a[:, 0].astype(np.int64) - 1, a[:, 1].astype(np.int64) - 1, a[:, 2])
+    # the server's figure was computed from couplings with more digits than
+    # the 12 the instance file keeps: agreement to 1e-7 per spin
+    assert abs(E / 1024 - (-1.55460615142578)) < 1e-7
+
+
+# ------------------------------------------------------------ fixed semantics: stationarity
+def _exact_mean_energy(nbs, P, T, jp):
+    """<(1/P) sum_k E_k> under exp(-A/T), A = sum_k E_k - Jp sum_k,i s^k s^{k+1},
+    by enumerating all 2^(N P) configurations."""
+    N = nbs.shape[0]
+    S = np.array(list(itertools.product([1, -1], repeat=N * P)), np.int8).reshape(-1, P, N)
+    Ek = np.zeros((S.shape[0], P))
+    for k in range(P):
+        Ek[:, k] = O.energy_nbs(S[:, k, :].copy(), nbs)
+    trot = sum((S[:, k, :].astype(float) * S[:, (k + 1) % P, :]).sum(1) for k in range(P))
+    A = Ek.sum(1) - jp * trot
+    w = np.exp(-(A - A.min()) / T)
+    return float((w * Ek.mean(1)).sum() / w.sum())
+
+
+def _small_frustrated():
+    # triangle 0-1-2 (frustrated, 3 colors) with fields
+    nbs = np.array([[[1, 1.0], [2, -1.0], [0, 0.3]],
+                    [[0, 1.0], [2, 1.0], [1, -0.2]],
+                    [[0, -1.0], [1, 1.0], [2, 0.1]]])
+    return nbs, np.array([0, 1, 2], np.int32), 3
+
+
+@pytest.mark.parametrize("gamma", [4.0, 6.0])
+def test_fixed_semantics_stationary_law(gamma):
+    nbs, color, nc = _small_frustrated()
+    P, T, steps = 4, 1.0, 200000
+    jp = O.jperp(gamma, P, T)
+    exact = _exact_mean_energy(nbs, P, T, jp)
+    sched = np.full(steps, gamma)
+    # colored sweep
+    sp = np.ones((P, 3), np.int8)
+    et = O.qa_colored(sched, 1, P, T, sp, nbs, color, nc, seed=11, trace=True)
+    # sequential sweep (random permutations, libc-style uniforms)
+    sp2 = np.ones((P, 3), np.int8)
+    perms = O.MT(2).perm_stream(3, steps)
+    _, et2 = O.qa_fixed_seq(sched, 1, P, T, sp2, nbs, perms, O.Type3(5), trace=True)
+    for trace in (et, et2):
+        x = trace[1000:]
+        b = x[: len(x) // 100 * 100].reshape(100, -1).mean(1)   # batch means
+        se = b.std(ddof=1) / np.sqrt(len(b))
+        assert abs(b.mean() - exact) < 5 * se + 1e-3, (b.mean(), exact, se)
+
+
+def test_colored_rejects_improper_coloring():
+    nbs, _, _ = _small_frustrated()
+    sp = np.ones((4, 3), np.int8)
+    with pytest.raises(ValueError):
+        O.qa_colored(np.array([1.0]), 1, 4, 1.0, sp, nbs, np.array([0, 1, 1], np.int32), 2, 1)
