@@ -109,3 +109,38 @@ def test_colored_full_size_properties(gpu):
     assert torch.all((c8 == 1) | (c8 == -1))
     np.testing.assert_allclose(et[:, -1].cpu().numpy(), E.mean(1).cpu().numpy(), rtol=0, atol=1e-9)
     assert float(E.min(1).values.mean()) / N < -1.3
+
+
+def test_colored_statistics_vs_sequential_oracle(gpu):
+    """Per-field-step mean energy and ground-state hit rate of the parallel
+    sweep (64 GPU replicas) within 3 sigma of the sequential fixed-semantics
+    sweep (64 oracle runs, random permutation order) on a quasi-static anneal
+    (20 sweeps per field step, so both orders sample the same law)."""
+    import torch
+    from piqmc import device
+    L, P, T, S, M, nseed = 6, 16, 0.1, 30, 20, 64
+    N, nbs, _, _, _ = pm_j_problem(L, seed=7)
+    sched = np.linspace(3.0, 1e-8, S)
+    pb = device.Problem(nbs)
+    starts = []
+    for s in range(nseed):
+        rng = np.random.RandomState(s)
+        starts.append(np.tile((2 * rng.randint(2, size=N) - 1).astype(np.int8), (P, 1)))
+    c8 = torch.tensor(np.stack(starts), device=gpu)
+    etg = pb.qa_colored(c8, sched, M, T, 99, trace=True).cpu().numpy()
+    Eg = pb.slice_energies(c8).cpu().numpy().min(1)
+    ets, Es = [], []
+    for s in range(nseed):
+        sp = starts[s].copy()
+        _, e2 = O.qa_fixed_seq(sched, M, P, T, sp, nbs, O.MT(1000 + s).perm_stream(N, S * M),
+                               O.Type3(5000 + s), trace=True)
+        ets.append(e2)
+        Es.append(O.energy_nbs(sp, nbs).min())
+    ets, Es = np.array(ets), np.array(Es)
+    se = np.sqrt(etg.var(0, ddof=1) / nseed + ets.var(0, ddof=1) / nseed)
+    z = (etg.mean(0) - ets.mean(0)) / np.maximum(se, 1e-12)
+    assert np.all(np.abs(z) < 3.0), z
+    e_best = min(Eg.min(), Es.min())
+    pg, ps = np.mean(Eg == e_best), np.mean(Es == e_best)
+    pp = (pg + ps) / 2
+    assert abs(pg - ps) <= 3 * np.sqrt(max(pp * (1 - pp), 1e-12) * 2 / nseed) + 1e-12, (pg, ps)

@@ -10,9 +10,10 @@ from conftest import init_confs, pm_j_problem
 pytestmark = pytest.mark.gpu
 
 
-def _float_instance(N, seed, fields=True, maxnb=None):
+def _float_instance(N, seed, fields=True, pad=2):
     """Random sparse graph with U[-2,2] couplings (the reference's generator
-    range, piqmc/tools.pyx:98-130), optional fields, ragged rows."""
+    range, piqmc/tools.pyx:98-130), optional fields, ragged rows (every row
+    padded with at least `pad` [0, 0] entries)."""
     from piqmc import instances
     rng = np.random.RandomState(seed)
     rows, cols = [], []
@@ -25,7 +26,8 @@ def _float_instance(N, seed, fields=True, maxnb=None):
     if fields:
         rows = np.concatenate([rows, np.arange(N)]); cols = np.concatenate([cols, np.arange(N)])
         vals = np.concatenate([vals, rng.uniform(-1, 1, N)])
-    return instances.neighbors(N, rows, cols, vals, maxnb=maxnb)
+    nbs = instances.neighbors(N, rows, cols, vals)
+    return np.concatenate([nbs, np.zeros((N, pad, 2))], axis=1)
 
 
 def _run_replay(gpu, nbs, P, sched, M, T, seeds, rseeds):
@@ -73,7 +75,7 @@ def test_qa_replay_pm_j(gpu, L, P, S, T):
 
 
 def test_qa_replay_float_couplings_fields_ragged(gpu):
-    nbs = _float_instance(45, 3, fields=True, maxnb=9)
+    nbs = _float_instance(45, 3, fields=True)
     _run_replay(gpu, nbs, 6, np.linspace(1.5, 1e-8, 12), 2, 0.01, seeds=[5, 6, 7], rseeds=[9, 10, 11])
 
 
@@ -93,7 +95,7 @@ def test_qa_replay_full_size_subset(gpu):
 def test_sa_replay(gpu):
     import torch
     from piqmc import device
-    nbs = _float_instance(70, 8, fields=True, maxnb=10)
+    nbs = _float_instance(70, 8, fields=True)
     N = nbs.shape[0]
     sched = np.linspace(3.0, 0.01, 20)
     M, seeds, rseeds = 2, [1, 2, 3, 4], [5, 6, 7, 8]
