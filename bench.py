@@ -62,7 +62,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--replicas", type=int, default=4096)
-    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0, help="threads per workgroup (0: autotune)")
+    ap.add_argument("--rpw", type=int, default=0, help="replicas per workgroup (0: autotune)")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="one anneal plus a 1 GiB device copy (for rocprofv3 --pmc runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-replay", action="store_true")
     ap.add_argument("--replay-steps", type=int, default=25)
@@ -86,10 +89,26 @@ def main():
     init = (torch.randint(0, 2, (R, P, N), generator=g, dtype=torch.int8) * 2 - 1).to(dev)
     confs = torch.empty_like(init)
     attempts = N * P * S * R
+    if args.block and args.rpw:
+        tile = (args.block, args.rpw)
+    else:
+        from piqmc import autotune
+        tile = autotune.tune_colored(pb, P, R, T, sched, use_cache=False)
+
+    if args.pmc_probe:
+        # one anneal, then a device copy of known size for counter calibration
+        confs.copy_(init)
+        pb.qa_colored(confs, sched, 1, T, 12345, block=tile[0], rpw=tile[1])
+        src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_probe": True, "copy_bytes": 1 << 30, "tile": tile}))
+        return
 
     def step():
         confs.copy_(init)
-        pb.qa_colored(confs, sched, 1, T, 12345, block=args.block)
+        pb.qa_colored(confs, sched, 1, T, 12345, block=tile[0], rpw=tile[1])
 
     for _ in range(args.warmup):
         step()
@@ -126,7 +145,7 @@ def main():
                    "replicas": R, "nspins": N, "slices": P, "field_steps": S, "sweeps_per_step": 1},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "colored_kernel<uint32,+-1 path>",
+                     "kernel": "colored_kernel<uint32,+-1 path>", "tile": {"block": tile[0], "rpw": tile[1]},
                      "kernel_ms": kavg * 1e3, "kernel_attempts_per_s": k_rate,
                      "bytes_per_attempt": BYTES_PER_ATTEMPT,
                      "lds_frac": achieved / LDS_PEAK_GBS},

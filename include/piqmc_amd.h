@@ -119,12 +119,13 @@ int piqmc_anneal(const double *sched, int S, int mcsteps, double *svec, int64_t 
 /*  confs_dev int8 [R][P][N] in/out (P even).                          */
 /*  etrace_dev float64 [R][S] (or NULL): mean over slices of the       */
 /*  classical energy after each field step.                            */
-/*  block: threads per workgroup (0 = tuned default).                  */
+/*  block: threads per workgroup (0 = default), rpw: replicas per      */
+/*  workgroup (0 = default); every choice gives identical results.     */
 /* ------------------------------------------------------------------ */
 int piqmc_qa_colored(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
                      const double *sched_host, int8_t *confs_dev, uint64_t seed,
                      uint32_t replica_base, uint32_t sweep_base, double *etrace_dev,
-                     int block, void *stream);
+                     int block, int rpw, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Energies.                                                           */

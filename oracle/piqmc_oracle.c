@@ -408,11 +408,30 @@ int orc_qa_fixed_seq(const double *sched, int S, int M, int P, float temp, int N
 /* ------------------------------------------------------------------ */
 /* Colored (parallel) fixed-semantics sweep: phases (tau, c) for tau in */
 /* {0,1} (slice parity) and c over the spatial colors; a phase updates  */
-/* every spin i of color c in every slice k = tau (mod 2).  The draw of */
-/* attempt (sweep g, spin i, slice k, replica r) is                    */
-/*   philox4x32_10(key = seed, ctr = {g, i, r, tau<<24 | (k>>1)>>2})[(k>>1)&3] */
-/* and it is accepted iff draw < thr of piqmc_fixed_class.             */
+/* every spin i of color c in every slice k = tau (mod 2).              */
+/* Draws are bit-sliced per (sweep g, spin i, replica r, parity tau):   */
+/* with w = 32 when P/2 <= 32 (else 64), random words R_0..R_31 of w    */
+/* bits come from philox4x32_10(key = seed, ctr = {g, i, r, tau<<24 | q}) */
+/*   w = 32: R_b = out[b&3] with q = b>>2                               */
+/*   w = 64: R_b = out[2(b&1)] | out[2(b&1)+1]<<32 with q = b>>1        */
+/* and the 32-bit uniform of slice k (bit m = k>>1) is                  */
+/*   u = sum_b ((R_b >> m) & 1) << (31 - b).                            */
+/* The attempt is accepted iff u < thr of piqmc_fixed_class.            */
 /* ------------------------------------------------------------------ */
+static void orc_sliced_words(uint32_t g, uint32_t i, uint32_t rep, uint32_t tau, int wide,
+                             const uint32_t *key, uint64_t *R)
+{
+    for (int q = 0; q < (wide ? 16 : 8); ++q) {
+        uint32_t ctr[4] = {g, i, rep, (tau << 24) | (uint32_t)q}, out[4];
+        orc_philox4x32(ctr, key, out);
+        if (wide) {
+            R[2 * q] = (uint64_t)out[0] | ((uint64_t)out[1] << 32);
+            R[2 * q + 1] = (uint64_t)out[2] | ((uint64_t)out[3] << 32);
+        } else {
+            for (int t = 0; t < 4; ++t) R[4 * q + t] = out[t];
+        }
+    }
+}
 int orc_qa_colored(const double *sched, int S, int M, int P, float temp, int N,
                    int8_t *sp, const double *nbs, int Z, const int32_t *color,
                    int ncolors, uint64_t seed, uint32_t replica, uint32_t sweep_base,
@@ -437,18 +456,24 @@ int orc_qa_colored(const double *sched, int S, int M, int P, float temp, int N,
                 for (int c = 0; c < ncolors; ++c) {
                     for (int i = 0; i < N; ++i) {
                         if (color[i] != c) continue;
+                        uint64_t R[32];
+                        int have_R = 0;
                         for (int k = tau; k < P; k += 2) {
                             double ed = orc_fixed_ediff(sp, P, N, k, i, nbs, Z, jperp);
                             unsigned int thr = 0;
                             int cls = piqmc_fixed_class(ed, (double)temp, &thr);
                             int flip = cls == 1;
                             if (cls == 2) {
-                                uint32_t mb = (uint32_t)(k >> 1);
-                                uint32_t ctr[4] = {g, (uint32_t)i, replica,
-                                                   ((uint32_t)tau << 24) | (mb >> 2)};
-                                uint32_t out[4];
-                                orc_philox4x32(ctr, key, out);
-                                flip = out[mb & 3u] < thr;
+                                if (!have_R) {
+                                    orc_sliced_words(g, (uint32_t)i, replica, (uint32_t)tau, P / 2 > 32,
+                                                     key, R);
+                                    have_R = 1;
+                                }
+                                int mb = k >> 1;
+                                uint32_t u = 0;
+                                for (int b = 0; b < 32; ++b)
+                                    u |= (uint32_t)((R[b] >> mb) & 1u) << (31 - b);
+                                flip = u < thr;
                             }
                             if (flip) sp[(size_t)k * N + i] = (int8_t)-sp[(size_t)k * N + i];
                         }

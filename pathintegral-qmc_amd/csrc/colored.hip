@@ -31,7 +31,7 @@ constexpr int kClassStride = 8 * 8 * 3;  // (z, u_s, u_t) thresholds per field s
 
 template <typename WT>
 struct ColArgs {
-    int N, Z, P, W, S, M, ncolors, zmax;
+    int N, Z, P, W, S, M, ncolors, zmax, R, rpw, zuni;
     float temp;
     uint32_t key0, key1, replica_base, sweep_base;
     const float *jperp;        // [S]
@@ -41,8 +41,7 @@ struct ColArgs {
     const int32_t *cnb;        // [zmax][N] (+-1 path)
     const uint32_t *cinfo;     // [N]
     const uint32_t *thr;       // [S][kClassStride]
-    const uint32_t *accm;      // [S][8]
-    const uint32_t *stom;      // [S][8]
+    const uint32_t *ztab;      // [S][8] per degree z: u_s thresholds (see run_colored)
     WT *packed;                // [R][2][N]
     double *etrace;            // [R][S] or null
 };
@@ -84,123 +83,204 @@ __device__ T block_sum(T v, T *red)
     return t;
 }
 
+// t-th bit-sliced random word of a Philox4x32 output block: four 32-bit words
+// for 32-bit spin words, two 64-bit words for 64-bit spin words.
+template <typename WT>
+__device__ __forceinline__ WT rword(const uint32_t out[4], int t)
+{
+    if constexpr (sizeof(WT) == 4) return out[t];
+    else return (WT)out[2 * t] | ((WT)out[2 * t + 1] << 32);
+}
+
+// Smallest-count masks of a bit-sliced counter (c2 c1 c0): bit set where the
+// count is >= k.  k is uniform across the wavefront on regular lattices.
+template <typename WT>
+__device__ __forceinline__ WT ge_pick(int k, WT c0, WT c1, WT c2)
+{
+    switch (k) {
+    case 0: return ~WT(0);
+    case 1: return c0 | c1 | c2;
+    case 2: return c1 | c2;
+    case 3: return c2 | (c1 & c0);
+    case 4: return c2;
+    case 5: return c2 & (c1 | c0);
+    case 6: return c2 & c1;
+    case 7: return c2 & c1 & c0;
+    default: return WT(0);
+    }
+}
+
 template <typename WT, bool PMJ>
 __global__ void colored_kernel(ColArgs<WT> a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int N = a.N, W = a.W, tid = threadIdx.x, bd = blockDim.x;
-    WT *st = reinterpret_cast<WT *>(smem);                       // [2][N]
-    uint32_t *tthr = reinterpret_cast<uint32_t *>(st + 2 * N);    // [kClassStride]
-    uint32_t *tacc = tthr + kClassStride;                         // [8]
-    uint32_t *tsto = tacc + 8;                                    // [8]
-    double *red = reinterpret_cast<double *>(tsto + 8);           // [16]
+    const int r0 = blockIdx.x * a.rpw;
+    const int nrep = min(a.rpw, a.R - r0);
+    WT *st = reinterpret_cast<WT *>(smem);                            // [nrep][2][N]
+    uint32_t *tthr = reinterpret_cast<uint32_t *>(st + (size_t)2 * N * a.rpw);  // [kClassStride]
+    uint32_t *tz = tthr + kClassStride;                               // [8]
+    double *red = reinterpret_cast<double *>(tz + 8);                 // [16]
     const WT valid = (W == (int)(8 * sizeof(WT))) ? ~WT(0) : ((WT(1) << W) - 1);
-    const int b = blockIdx.x;
-    const uint32_t rep = a.replica_base + (uint32_t)b;
-    WT *gst = a.packed + (size_t)b * 2 * N;
-    for (int q = tid; q < 2 * N; q += bd) st[q] = gst[q];
+    WT *gst = a.packed + (size_t)r0 * 2 * N;
+    for (int q = tid; q < 2 * N * nrep; q += bd) st[q] = gst[q];
 
     for (int s = 0; s < a.S; ++s) {
         __syncthreads();
         if (PMJ) {
             for (int q = tid; q < kClassStride; q += bd) tthr[q] = a.thr[(size_t)s * kClassStride + q];
-            if (tid < 8) {
-                tacc[tid] = a.accm[s * 8 + tid];
-                tsto[tid] = a.stom[s * 8 + tid];
-            }
+            if (tid < 8) tz[tid] = a.ztab[s * 8 + tid];
         }
         __syncthreads();
+        // uniform-degree lattices: the class thresholds of this step are wave-uniform
+        const uint32_t zt_u = (PMJ && a.zuni >= 0) ? a.ztab[s * 8 + a.zuni] : 0u;
         const float jp = a.jperp[s];
         for (int m = 0; m < a.M; ++m) {
             const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
             for (int tau = 0; tau < 2; ++tau) {
-                WT *mine = st + tau * N;
-                const WT *other = st + (1 - tau) * N;
                 for (int c = 0; c < a.ncolors; ++c) {
                     const int q0 = a.color_off[c], q1 = a.color_off[c + 1];
                     for (int q = q0 + tid; q < q1; q += bd) {
                         const int i = a.order[q];
-                        const WT S_ = mine[i], O = other[i];
-                        const WT Lw = tau == 0 ? rotl1(O, W, valid) : O;
-                        const WT Rw = tau == 0 ? O : rotr1(O, W);
-                        const WT uL = S_ ^ Lw, uR = S_ ^ Rw;  // 1 = antiparallel Trotter bond
-                        WT flips = 0;
+                        uint32_t info = 0;
+                        int zi = 0;
+                        int nb[8];
                         if (PMJ) {
-                            const uint32_t info = a.cinfo[q];
-                            const int zi = (info >> 8) & 15;
-                            WT c0 = 0, c1 = 0, c2 = 0;  // bit-sliced unsatisfied-bond counter
-                            for (int z = 0; z < a.zmax; ++z) {
-                                if (z < zi) {
-                                    const int j = a.cnb[(size_t)z * N + q];
-                                    const WT x = S_ ^ mine[j] ^ (((info >> z) & 1u) ? ~WT(0) : WT(0));
-                                    const WT k0 = c0 & x;
-                                    c0 ^= x;
-                                    const WT k1 = c1 & k0;
-                                    c1 ^= k0;
-                                    c2 ^= k1;
-                                }
-                            }
-                            const uint32_t am = tacc[zi], sm = tsto[zi];
-                            const WT et[3] = {~(uL | uR), uL ^ uR, uL & uR};
-                            WT X = 0;
-                            for (int us = 0; us <= a.zmax; ++us) {
-                                const WT es = ((us & 4) ? c2 : ~c2) & ((us & 2) ? c1 : ~c1) & ((us & 1) ? c0 : ~c0);
+                            info = a.cinfo[q];
+                            zi = (info >> 8) & 15;
 #pragma unroll
-                                for (int ut = 0; ut < 3; ++ut) {
-                                    const int bp = us * 3 + ut;
-                                    if ((am >> bp) & 1u) flips |= es & et[ut];
-                                    else if ((sm >> bp) & 1u) X |= es & et[ut];
-                                }
-                            }
-                            flips &= valid;
-                            X &= valid;
-                            while (X) {
-                                const int qd = ctz(X) >> 2;
-                                uint32_t out[4];
-                                philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd, a.key0,
-                                              a.key1, out);
-                                const WT qm = (X >> (4 * qd)) & WT(15);
-#pragma unroll
-                                for (int bb = 0; bb < 4; ++bb) {
-                                    if ((qm >> bb) & WT(1)) {
-                                        const int mm = 4 * qd + bb;
-                                        const int us = (int)(((c2 >> mm) & 1) << 2 | ((c1 >> mm) & 1) << 1 | ((c0 >> mm) & 1));
-                                        const int ut = (int)((uL >> mm) & 1) + (int)((uR >> mm) & 1);
-                                        if (out[bb] < tthr[(zi * 8 + us) * 3 + ut]) flips |= WT(1) << mm;
-                                    }
-                                }
-                                X &= ~(WT(15) << (4 * qd));
-                            }
-                        } else {
-                            const int2 *row = a.rows + (size_t)i * a.Z;
-                            int qcur = -1;
-                            uint32_t out[4] = {0, 0, 0, 0};
-                            for (int mm = 0; mm < W; ++mm) {
-                                const double si = ((S_ >> mm) & 1) ? -1.0 : 1.0;
-                                double h = 0.0;
-                                for (int e = 0; e < a.Z; ++e) {
-                                    const int2 rw = row[e];
-                                    const double J = (double)__int_as_float(rw.y);
-                                    if (rw.x == i) h += J;
-                                    else h += J * (((mine[rw.x] >> mm) & 1) ? -1.0 : 1.0);
-                                }
-                                const double ts = (((Lw >> mm) & 1) ? -1.0 : 1.0) + (((Rw >> mm) & 1) ? -1.0 : 1.0);
-                                const double ed = -2.0 * si * (h + (double)jp * ts);
-                                unsigned int thr = 0;
-                                const int cls = piqmc_fixed_class(ed, (double)a.temp, &thr);
-                                bool fl = cls == 1;
-                                if (cls == 2) {
-                                    if ((mm >> 2) != qcur) {
-                                        qcur = mm >> 2;
-                                        philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qcur,
-                                                      a.key0, a.key1, out);
-                                    }
-                                    fl = out[mm & 3] < thr;
-                                }
-                                if (fl) flips |= WT(1) << mm;
-                            }
+                            for (int z = 0; z < 8; ++z)
+                                if (z < a.zmax) nb[z] = z < zi ? a.cnb[(size_t)z * N + q] : i;
                         }
-                        mine[i] = S_ ^ flips;
+                        const uint32_t zt = (PMJ && a.zuni >= 0) ? zt_u : (PMJ ? tz[zi] : 0u);
+                        for (int rr = 0; rr < nrep; ++rr) {
+                            WT *mine = st + ((size_t)rr * 2 + tau) * N;
+                            const WT *other = st + ((size_t)rr * 2 + 1 - tau) * N;
+                            const uint32_t rep = a.replica_base + (uint32_t)(r0 + rr);
+                            const WT S_ = mine[i], O = other[i];
+                            const WT Lw = tau == 0 ? rotl1(O, W, valid) : O;
+                            const WT Rw = tau == 0 ? O : rotr1(O, W);
+                            const WT uL = S_ ^ Lw, uR = S_ ^ Rw;  // 1 = antiparallel Trotter bond
+                            WT flips = 0;
+                            if (PMJ) {
+                                WT c0 = 0, c1 = 0, c2 = 0;  // bit-sliced unsatisfied-bond counter
+#pragma unroll
+                                for (int z = 0; z < 8; ++z) {
+                                    if (z < a.zmax && z < zi) {
+                                        const WT x = S_ ^ mine[nb[z]] ^ (((info >> z) & 1u) ? ~WT(0) : WT(0));
+                                        const WT k0 = c0 & x;
+                                        c0 ^= x;
+                                        const WT k1 = c1 & k0;
+                                        c1 ^= k0;
+                                        c2 ^= k1;
+                                    }
+                                }
+                                const WT e0 = ~(uL | uR), e1 = uL ^ uR, e2 = uL & uR;
+                                flips = (e0 & ge_pick((int)(zt & 15), c0, c1, c2)) |
+                                        (e1 & ge_pick((int)((zt >> 8) & 15), c0, c1, c2)) |
+                                        (e2 & ge_pick((int)((zt >> 16) & 15), c0, c1, c2));
+                                WT X = (e0 & ge_pick((int)((zt >> 4) & 15), c0, c1, c2)) |
+                                       (e1 & ge_pick((int)((zt >> 12) & 15), c0, c1, c2)) |
+                                       (e2 & ge_pick((int)((zt >> 20) & 15), c0, c1, c2));
+                                flips &= valid;
+                                X &= valid & ~flips;
+                                if (X) {
+                                    // Bit-sliced comparison u < thr for every stochastic bit at
+                                    // once, most significant uniform bit first: random word R_b
+                                    // supplies bit 31-b of all uniforms; a bit is decided at the
+                                    // first b where its uniform bit differs from its threshold bit.
+                                    WT lt = 0, und = X;
+                                    // The stochastic classes of this word (one per field step on
+                                    // a regular lattice, rarely more): masks and thresholds in
+                                    // registers; a fourth class falls back to the table walk.
+                                    WT cm0 = 0, cm1 = 0, cm2 = 0;
+                                    uint32_t ct0 = 0, ct1 = 0, ct2 = 0;
+                                    int ns = 0;
+#pragma unroll
+                                    for (int ut = 0; ut < 3; ++ut) {
+                                        const WT eu = ut == 0 ? e0 : (ut == 1 ? e1 : e2);
+                                        const int amin = (int)((zt >> (8 * ut)) & 15);
+                                        for (int us = (int)((zt >> (8 * ut + 4)) & 15); us < amin; ++us) {
+                                            const WT mk = eu & ge_pick(us, c0, c1, c2) & ~ge_pick(us + 1, c0, c1, c2);
+                                            const uint32_t tk = tthr[(zi * 8 + us) * 3 + ut];
+                                            if (ns == 0) { cm0 = mk; ct0 = tk; }
+                                            else if (ns == 1) { cm1 = mk; ct1 = tk; }
+                                            else if (ns == 2) { cm2 = mk; ct2 = tk; }
+                                            ++ns;
+                                        }
+                                    }
+                                    constexpr int kPer = sizeof(WT) == 4 ? 4 : 2;  // words per Philox call
+                                    for (int qd = 0; qd < 32 / kPer && und; ++qd) {
+                                        uint32_t out[4];
+                                        philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd,
+                                                      a.key0, a.key1, out);
+#pragma unroll
+                                        for (int t = 0; t < kPer; ++t) {
+                                            const WT Rb = rword<WT>(out, t);
+                                            const int sh = 31 - (qd * kPer + t);
+                                            WT tb;  // bits whose class threshold has bit sh set
+                                            if (ns <= 3) {
+                                                tb = (((ct0 >> sh) & 1u) ? cm0 : WT(0)) | (((ct1 >> sh) & 1u) ? cm1 : WT(0)) |
+                                                     (((ct2 >> sh) & 1u) ? cm2 : WT(0));
+                                            } else {
+                                                tb = 0;
+#pragma unroll
+                                                for (int ut = 0; ut < 3; ++ut) {
+                                                    const WT eu = ut == 0 ? e0 : (ut == 1 ? e1 : e2);
+                                                    const int amin = (int)((zt >> (8 * ut)) & 15);
+                                                    for (int us = (int)((zt >> (8 * ut + 4)) & 15); us < amin; ++us)
+                                                        if ((tthr[(zi * 8 + us) * 3 + ut] >> sh) & 1u)
+                                                            tb |= eu & ge_pick(us, c0, c1, c2) & ~ge_pick(us + 1, c0, c1, c2);
+                                                }
+                                            }
+                                            lt |= und & ~Rb & tb;
+                                            und &= ~(Rb ^ tb);
+                                        }
+                                    }
+                                    flips |= lt & X;
+                                }
+                            } else {
+                                const int2 *row = a.rows + (size_t)i * a.Z;
+                                bool have_r = false;
+                                WT R[32];
+                                for (int mm = 0; mm < W; ++mm) {
+                                    const double si = ((S_ >> mm) & 1) ? -1.0 : 1.0;
+                                    double h = 0.0;
+                                    for (int e = 0; e < a.Z; ++e) {
+                                        const int2 rw = row[e];
+                                        const double J = (double)__int_as_float(rw.y);
+                                        if (rw.x == i) h += J;
+                                        else h += J * (((mine[rw.x] >> mm) & 1) ? -1.0 : 1.0);
+                                    }
+                                    const double ts = (((Lw >> mm) & 1) ? -1.0 : 1.0) + (((Rw >> mm) & 1) ? -1.0 : 1.0);
+                                    const double ed = -2.0 * si * (h + (double)jp * ts);
+                                    unsigned int thr = 0;
+                                    const int cls = piqmc_fixed_class(ed, (double)a.temp, &thr);
+                                    bool fl = cls == 1;
+                                    if (cls == 2) {
+                                        if (!have_r) {  // the word's 32 bit-sliced random words
+                                            have_r = true;
+                                            constexpr int kPer = sizeof(WT) == 4 ? 4 : 2;
+#pragma unroll
+                                            for (int qd = 0; qd < 32 / kPer; ++qd) {
+                                                uint32_t out[4];
+                                                philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd,
+                                                              a.key0, a.key1, out);
+#pragma unroll
+                                                for (int t = 0; t < kPer; ++t)
+                                                    R[qd * kPer + t] = rword<WT>(out, t);
+                                            }
+                                        }
+                                        uint32_t u = 0;
+#pragma unroll
+                                        for (int bb = 0; bb < 32; ++bb) u |= (uint32_t)((R[bb] >> mm) & 1) << (31 - bb);
+                                        fl = u < thr;
+                                    }
+                                    if (fl) flips |= WT(1) << mm;
+                                }
+                            }
+                            mine[i] = S_ ^ flips;
+                        }
                     }
                     __syncthreads();
                 }
@@ -209,28 +289,32 @@ __global__ void colored_kernel(ColArgs<WT> a)
         if (a.etrace) {
             // Mean over slices of the classical energy; sums of +-1 products are
             // popcounts of XORed parity words.
-            double acc = 0.0;
-            for (int i = tid; i < N; i += bd) {
-                const WT a0 = st[i], a1 = st[N + i];
-                const int2 *row = a.rows + (size_t)i * a.Z;
-                for (int e = 0; e < a.Z; ++e) {
-                    const int2 rw = row[e];
-                    const double J = (double)__int_as_float(rw.y);
-                    if (J == 0.0) continue;
-                    if (rw.x == i) {
-                        acc -= J * (double)(a.P - 2 * (popc(a0) + popc(a1)));
-                    } else {
-                        const int pc = popc(a0 ^ st[rw.x]) + popc(a1 ^ st[N + rw.x]);
-                        acc -= 0.5 * J * (double)(a.P - 2 * pc);
+            for (int rr = 0; rr < nrep; ++rr) {
+                const WT *s0 = st + (size_t)rr * 2 * N, *s1 = s0 + N;
+                double acc = 0.0;
+                for (int i = tid; i < N; i += bd) {
+                    const WT a0 = s0[i], a1 = s1[i];
+                    const int2 *row = a.rows + (size_t)i * a.Z;
+                    for (int e = 0; e < a.Z; ++e) {
+                        const int2 rw = row[e];
+                        const double J = (double)__int_as_float(rw.y);
+                        if (J == 0.0) continue;
+                        if (rw.x == i) {
+                            acc -= J * (double)(a.P - 2 * (popc(a0) + popc(a1)));
+                        } else {
+                            const int pc = popc(a0 ^ s0[rw.x]) + popc(a1 ^ s1[rw.x]);
+                            acc -= 0.5 * J * (double)(a.P - 2 * pc);
+                        }
                     }
                 }
+                const double tot = block_sum(acc, red);
+                if (tid == 0) a.etrace[(size_t)(r0 + rr) * a.S + s] = tot / (double)a.P;
+                __syncthreads();
             }
-            const double tot = block_sum(acc, red);
-            if (tid == 0) a.etrace[(size_t)b * a.S + s] = tot / (double)a.P;
         }
     }
     __syncthreads();
-    for (int q = tid; q < 2 * N; q += bd) gst[q] = st[q];
+    for (int q = tid; q < 2 * N * nrep; q += bd) gst[q] = st[q];
 }
 
 // int8 [R][P][N] -> packed [R][2][N] (bit m of parity tau = slice 2m+tau, set = -1)
@@ -274,48 +358,58 @@ int grid_for(size_t n, int block)
 template <typename WT>
 int run_colored(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
                 const std::vector<float> &jp, int8_t *confs, uint64_t seed, uint32_t rbase,
-                uint32_t sbase, double *etrace, int block, hipStream_t st)
+                uint32_t sbase, double *etrace, int block, int rpw, hipStream_t st)
 {
     const int N = pb->N;
-    const bool pmj = pb->pmj != 0;
-    const size_t lds = sizeof(WT) * 2 * (size_t)N + 4 * (kClassStride + 16) + 16 * sizeof(double);
-    if (lds > 160 * 1024)
-        return piqmc::set_error(PIQMC_EINVAL, "colored sweep: 2*nspins*P/2 bits exceed one workgroup's LDS");
-    // Per-field-step tables for the +-1 path.
-    std::vector<uint32_t> thr, accm, stom;
+    bool pmj = pb->pmj != 0;
+    // Per-field-step tables for the +-1 path.  The decision class of a bit is
+    // a function of (z, u_s, u_t) and, at fixed (z, u_t), monotone in the
+    // unsatisfied-bond count u_s (ediff grows by 4 per unit), so each (z, u_t)
+    // is summarised by two thresholds: accept for u_s >= amin, draw for
+    // smin <= u_s < amin, reject below.  ztab packs amin/smin of u_t = 0,1,2
+    // in nibbles (amin, smin) at bits 0,4 / 8,12 / 16,20.
+    std::vector<uint32_t> thr, ztab;
     if (pmj) {
         thr.assign((size_t)S * kClassStride, 0);
-        accm.assign((size_t)S * 8, 0);
-        stom.assign((size_t)S * 8, 0);
-        for (int s = 0; s < S; ++s)
-            for (int z = 0; z <= pb->zmax; ++z)
-                for (int us = 0; us <= z; ++us)
-                    for (int ut = 0; ut < 3; ++ut) {
+        ztab.assign((size_t)S * 8, 0);
+        for (int s = 0; s < S && pmj; ++s)
+            for (int z = 0; z <= pb->zmax && pmj; ++z)
+                for (int ut = 0; ut < 3 && pmj; ++ut) {
+                    int cls[9];
+                    for (int us = 0; us <= z; ++us) {
                         // -2 s_i (h + Jp*ts) with s_i*h = z - 2u_s, s_i*ts = 2 - 2u_t (exact)
                         const double ed = -2.0 * ((double)(z - 2 * us) + (double)jp[s] * (double)(2 - 2 * ut));
                         unsigned int t = 0;
-                        const int cls = piqmc_fixed_class(ed, (double)temp, &t);
-                        const int bp = us * 3 + ut;
-                        if (cls == 1) accm[(size_t)s * 8 + z] |= 1u << bp;
-                        if (cls == 2) {
-                            stom[(size_t)s * 8 + z] |= 1u << bp;
-                            thr[(size_t)s * kClassStride + (z * 8 + us) * 3 + ut] = t;
-                        }
+                        cls[us] = piqmc_fixed_class(ed, (double)temp, &t);
+                        if (cls[us] == 2) thr[(size_t)s * kClassStride + (z * 8 + us) * 3 + ut] = t;
                     }
+                    int amin = z + 1, smin = z + 1;
+                    for (int us = z; us >= 0 && cls[us] == 1; --us) amin = us;
+                    for (int us = amin - 1; us >= 0 && cls[us] == 2; --us) smin = us;
+                    if (smin > amin) smin = amin;
+                    for (int us = 0; us < smin; ++us)
+                        if (cls[us] != 0) pmj = false;  // not monotone: use the per-bit path
+                    ztab[(size_t)s * 8 + z] |= ((uint32_t)amin | ((uint32_t)smin << 4)) << (8 * ut);
+                }
     }
+    if (rpw <= 0) rpw = R >= 2048 ? 4 : (R >= 1024 ? 2 : 1);
+    if (rpw > 64) return piqmc::set_error(PIQMC_EINVAL, "replicas per workgroup must be <= 64");
+    const size_t lds = sizeof(WT) * 2 * (size_t)N * rpw + 4 * (kClassStride + 8) + 16 * sizeof(double);
+    if (lds > 160 * 1024)
+        return piqmc::set_error(PIQMC_EINVAL, "colored sweep: replicas per workgroup x 2 x nspins x P/2 bits exceed one workgroup's LDS");
+    if (block <= 0) block = 256;
+    if (block > 1024 || (block & 63)) return piqmc::set_error(PIQMC_EINVAL, "block must be a multiple of 64, <= 1024");
     float *d_jp = nullptr;
-    uint32_t *d_thr = nullptr, *d_acc = nullptr, *d_sto = nullptr;
+    uint32_t *d_thr = nullptr, *d_zt = nullptr;
     int *d_coff = nullptr;
     WT *d_packed = nullptr;
     PIQMC_HIP(hipMallocAsync((void **)&d_jp, sizeof(float) * (S ? S : 1), st));
     if (S) PIQMC_HIP(hipMemcpyAsync(d_jp, jp.data(), sizeof(float) * S, hipMemcpyHostToDevice, st));
     if (pmj && S) {
         PIQMC_HIP(hipMallocAsync((void **)&d_thr, sizeof(uint32_t) * thr.size(), st));
-        PIQMC_HIP(hipMallocAsync((void **)&d_acc, sizeof(uint32_t) * accm.size(), st));
-        PIQMC_HIP(hipMallocAsync((void **)&d_sto, sizeof(uint32_t) * stom.size(), st));
+        PIQMC_HIP(hipMallocAsync((void **)&d_zt, sizeof(uint32_t) * ztab.size(), st));
         PIQMC_HIP(hipMemcpyAsync(d_thr, thr.data(), sizeof(uint32_t) * thr.size(), hipMemcpyHostToDevice, st));
-        PIQMC_HIP(hipMemcpyAsync(d_acc, accm.data(), sizeof(uint32_t) * accm.size(), hipMemcpyHostToDevice, st));
-        PIQMC_HIP(hipMemcpyAsync(d_sto, stom.data(), sizeof(uint32_t) * stom.size(), hipMemcpyHostToDevice, st));
+        PIQMC_HIP(hipMemcpyAsync(d_zt, ztab.data(), sizeof(uint32_t) * ztab.size(), hipMemcpyHostToDevice, st));
     }
     PIQMC_HIP(hipMallocAsync((void **)&d_coff, sizeof(int) * (pb->ncolors + 1), st));
     PIQMC_HIP(hipMemcpyAsync(d_coff, pb->color_off.data(), sizeof(int) * (pb->ncolors + 1), hipMemcpyHostToDevice, st));
@@ -325,20 +419,20 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
     ColArgs<WT> a;
     a.N = N; a.Z = pb->Z; a.P = P; a.W = P / 2; a.S = S; a.M = M;
     a.ncolors = pb->ncolors; a.zmax = pb->zmax; a.temp = temp;
+    a.R = R; a.rpw = rpw; a.zuni = pmj ? pb->zuni : -1;
     a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
     a.replica_base = rbase; a.sweep_base = sbase;
     a.jperp = d_jp; a.rows = pb->d_rows; a.order = pb->d_order; a.color_off = d_coff;
-    a.cnb = pb->d_cnb; a.cinfo = pb->d_cinfo; a.thr = d_thr; a.accm = d_acc; a.stom = d_sto;
+    a.cnb = pb->d_cnb; a.cinfo = pb->d_cinfo; a.thr = d_thr; a.ztab = d_zt;
     a.packed = d_packed; a.etrace = etrace;
-    if (block <= 0) block = 256;
-    if (block > 1024 || (block & 63)) return piqmc::set_error(PIQMC_EINVAL, "block must be a multiple of 64, <= 1024");
+    const int grid = (R + rpw - 1) / rpw;
     piqmc::timing_begin(st);
     if (pmj) {
         (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        colored_kernel<WT, true><<<R, block, lds, st>>>(a);
+        colored_kernel<WT, true><<<grid, block, lds, st>>>(a);
     } else {
         (void)hipFuncSetAttribute((const void *)colored_kernel<WT, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        colored_kernel<WT, false><<<R, block, lds, st>>>(a);
+        colored_kernel<WT, false><<<grid, block, lds, st>>>(a);
     }
     piqmc::timing_end(st);
     PIQMC_HIP(hipGetLastError());
@@ -346,8 +440,7 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
     PIQMC_HIP(hipFreeAsync(d_packed, st));
     PIQMC_HIP(hipFreeAsync(d_coff, st));
     if (d_thr) PIQMC_HIP(hipFreeAsync(d_thr, st));
-    if (d_acc) PIQMC_HIP(hipFreeAsync(d_acc, st));
-    if (d_sto) PIQMC_HIP(hipFreeAsync(d_sto, st));
+    if (d_zt) PIQMC_HIP(hipFreeAsync(d_zt, st));
     PIQMC_HIP(hipFreeAsync(d_jp, st));
     PIQMC_HIP(hipGetLastError());
     return PIQMC_OK;
@@ -357,7 +450,7 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
 
 extern "C" int piqmc_qa_colored(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
                                 const double *sched, int8_t *confs, uint64_t seed, uint32_t rbase,
-                                uint32_t sbase, double *etrace, int block, void *stream)
+                                uint32_t sbase, double *etrace, int block, int rpw, void *stream)
 {
     if (!pb || R < 1 || S < 0 || M < 0 || (S > 0 && !sched) || !confs)
         return piqmc::set_error(PIQMC_EINVAL, "qa_colored: bad arguments");
@@ -371,6 +464,6 @@ extern "C" int piqmc_qa_colored(const piqmc_problem *pb, int R, int P, int S, in
     if ((long)S * M > 0 && temp == 0.0f) return piqmc::set_error(PIQMC_EZERODIV, "float division");
     hipStream_t st = (hipStream_t)stream;
     if (P / 2 <= 32)
-        return run_colored<uint32_t>(pb, R, P, S, M, temp, jp, confs, seed, rbase, sbase, etrace, block, st);
-    return run_colored<uint64_t>(pb, R, P, S, M, temp, jp, confs, seed, rbase, sbase, etrace, block, st);
+        return run_colored<uint32_t>(pb, R, P, S, M, temp, jp, confs, seed, rbase, sbase, etrace, block, rpw, st);
+    return run_colored<uint64_t>(pb, R, P, S, M, temp, jp, confs, seed, rbase, sbase, etrace, block, rpw, st);
 }

@@ -316,6 +316,13 @@ extern "C" int piqmc_problem_create(int N, int Z, const double *nbs, piqmc_probl
         p->zmax = std::max(p->zmax, z);
     }
     if (p->zmax > 7) p->pmj = 0;  // bit-sliced counter holds 0..7
+    p->zuni = p->zmax;
+    for (int i = 0; i < N && p->zuni >= 0; ++i) {
+        int z = 0;
+        for (int e = 0; e < Z; ++e)
+            if (p->J[(size_t)i * Z + e] != 0.0f) ++z;
+        if (z != p->zmax) p->zuni = -1;
+    }
     std::vector<int2> rows((size_t)N * Z);
     for (size_t t = 0; t < rows.size(); ++t) {
         rows[t].x = p->idx[t];

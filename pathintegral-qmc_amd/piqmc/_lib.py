@@ -38,7 +38,8 @@ _SIGS = {
     "piqmc_quantum_anneal": ([_vp, _i, _i, _i, _f, _i, _vp, _i64, _i64, _vp, _i, _vp, _vp, _vp,
                               _vp], _i),
     "piqmc_anneal": ([_vp, _i, _i, _vp, _i64, _i, _vp, _i, _vp, _vp, _vp, _vp], _i),
-    "piqmc_qa_colored": ([_vp, _i, _i, _i, _i, _f, _vp, _vp, _u64, _u32, _u32, _vp, _i, _vp], _i),
+    "piqmc_qa_colored": ([_vp, _i, _i, _i, _i, _f, _vp, _vp, _u64, _u32, _u32, _vp, _i, _i, _vp],
+                         _i),
     "piqmc_slice_energies": ([_vp, _i, _vp, _vp, _vp], _i),
     "piqmc_classical_energy": ([_i, _i64, _vp, _vp, _vp, _i, _vp, _vp], _i),
     "piqmc_generate_neighbors": ([_i, _i, _i64, _vp, _vp, _vp, _i, _vp], _i),

@@ -142,7 +142,7 @@ class Problem:
 
     # ------------------------------------------------------------ parallel
     def qa_colored(self, confs, sched, mcsteps, temp, seed, replica_base=0, sweep_base=0,
-                   trace=False, block=0, stream=None):
+                   trace=False, block=0, rpw=0, stream=None):
         """Parallel sweep of confs int8 [R,P,N] in place.  Returns the
         per-field-step mean slice energy float64 [R,S] when trace=True."""
         import torch
@@ -154,7 +154,7 @@ class Problem:
         _lib.call("piqmc_qa_colored", self._h, R, P, S, int(mcsteps), float(temp),
                   sched.ctypes.data, confs.data_ptr(), int(seed) & 0xFFFFFFFFFFFFFFFF,
                   int(replica_base), int(sweep_base), et.data_ptr() if trace else None,
-                  int(block), _stream(stream))
+                  int(block), int(rpw), _stream(stream))
         return et
 
     # ------------------------------------------------------------ energies

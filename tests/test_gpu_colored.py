@@ -13,7 +13,7 @@ from conftest import pm_j_problem
 pytestmark = pytest.mark.gpu
 
 
-def _run(gpu, nbs, P, sched, M, T, R, seed, rbase=0, sbase=0, block=0, coloring=None):
+def _run(gpu, nbs, P, sched, M, T, R, seed, rbase=0, sbase=0, block=0, rpw=0, coloring=None):
     import torch
     from piqmc import device
     N = nbs.shape[0]
@@ -25,7 +25,7 @@ def _run(gpu, nbs, P, sched, M, T, R, seed, rbase=0, sbase=0, block=0, coloring=
     c0 = (2 * rng.randint(2, size=(R, P, N)) - 1).astype(np.int8)
     c8 = torch.tensor(c0, device=gpu)
     et = pb.qa_colored(c8, sched, M, T, seed, replica_base=rbase, sweep_base=sbase, trace=True,
-                       block=block)
+                       block=block, rpw=rpw)
     torch.cuda.synchronize()
     got, etg = c8.cpu().numpy(), et.cpu().numpy()
     ref, etr = [], []
@@ -80,14 +80,16 @@ def test_colored_general_path_frustrated(gpu):
     np.testing.assert_allclose(etg, etr, rtol=1e-12, atol=1e-12)
 
 
-def test_colored_block_size_invariance(gpu):
-    # every workgroup size takes identical decisions (pinning any tuned tile)
-    N, nbs, _, _, _ = pm_j_problem(16, seed=3)
+@pytest.mark.parametrize("field_sigma", [0.0, 0.2])
+def test_colored_tile_invariance(gpu, field_sigma):
+    # every workgroup size and replicas-per-workgroup choice takes identical
+    # decisions (this pins every tile the autotuner may pick), on both paths
+    N, nbs, _, _, _ = pm_j_problem(16 if not field_sigma else 6, seed=3, field_sigma=field_sigma)
     outs = []
-    for block in (64, 128, 256, 512, 1024):
-        _, got, ref, _, _ = _run(gpu, nbs, 32, np.linspace(3.0, 1e-8, 12), 1, 1 / 32, R=3, seed=77,
-                                 block=block)
-        assert np.array_equal(got, ref)
+    for block, rpw in [(64, 1), (128, 2), (256, 3), (512, 4), (1024, 1), (256, 5)]:
+        _, got, ref, _, _ = _run(gpu, nbs, 32, np.linspace(3.0, 1e-8, 12), 1, 1 / 32, R=5, seed=77,
+                                 block=block, rpw=rpw)
+        assert np.array_equal(got, ref), (block, rpw)
         outs.append(got)
     assert all(np.array_equal(outs[0], o) for o in outs)
 
