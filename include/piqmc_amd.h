@@ -140,6 +140,43 @@ int piqmc_classical_energy(int nspins, int64_t nnz, const int64_t *rows, const i
                            const double *vals, int B, const double *spins, double *E);
 
 /* ------------------------------------------------------------------ */
+/* Learned slice proposal (no reference counterpart: the reference has */
+/* no learned component).  A masked autoregressive network (MADE)      */
+/* N -> H1 -> H2 -> N over one slice; BF16 MFMA kernels, FP32 accum.    */
+/* Slices are int8 [B][N] device arrays.                               */
+/* ------------------------------------------------------------------ */
+typedef struct piqmc_made piqmc_made;
+int piqmc_made_create(int nspins, int h1, int h2, uint64_t seed, piqmc_made **out);
+int piqmc_made_destroy(piqmc_made *m);
+/* fp32 parameters / last gradients to host: w0 [h1][N], b0 [h1], w1 [h2][h1],
+ * b1 [h2], w2 [N][h2], b2 [N] (any pointer may be NULL). */
+int piqmc_made_get_params(const piqmc_made *m, float *w0, float *b0, float *w1, float *b1,
+                          float *w2, float *b2);
+int piqmc_made_get_grads(const piqmc_made *m, float *w0, float *b0, float *w1, float *b1,
+                         float *w2, float *b2);
+/* log q of B slices -> logq_dev float64 [B]. */
+int piqmc_made_logprob(piqmc_made *m, int B, const int8_t *s_dev, double *logq_dev, void *stream);
+/* Gradient of the mean negative log-likelihood over B slices; an Adam step
+ * (beta 0.9/0.999, eps 1e-8) when lr > 0; *mean_nll (host, may be NULL). */
+int piqmc_made_train_step(piqmc_made *m, int B, const int8_t *s_dev, float lr, double *mean_nll,
+                          void *stream);
+/* B autoregressive draws -> int8 [B][N] and their log q (float64 [B]). */
+int piqmc_made_sample(piqmc_made *m, int B, uint64_t seed, uint32_t round, int8_t *s_out_dev,
+                      double *logq_dev, void *stream);
+/* Independence Metropolis-Hastings slice move for R replicas of confs_dev
+ * int8 [R][P][N]: replica r proposes new_dev[r] (log q in logq_new_dev[r])
+ * for slice k = slice_dev[r]; accepted with min(1, exp(-dS/T) q(old)/q(new)),
+ * dS the fixed-semantics action change (classical energy of the slice and
+ * its Trotter bonds to k-1, k+1 at this field).  accepted_dev int32 [R]. */
+int piqmc_made_mh(piqmc_made *m, const piqmc_problem *pb, int R, int P, int8_t *confs_dev,
+                  const int8_t *new_dev, const double *logq_new_dev, const int32_t *slice_dev,
+                  float jperp, float temp, uint64_t seed, uint32_t round, int32_t *accepted_dev,
+                  void *stream);
+/* C[M][N] = A[M][K] . B[N][K]^T, bf16 (uint16 bit patterns) in, fp32 out. */
+int piqmc_gemm_bf16_nt(const uint16_t *A, const uint16_t *B, int M, int N, int K, float *C,
+                       void *stream);
+
+/* ------------------------------------------------------------------ */
 /* tools.GenerateNeighbors (piqmc/tools.pyx:28-96).                    */
 /*  (rows, cols, vals) list the DOK keys in insertion order.           */
 /*  order 0: CPython 2.7 dict iteration order (the reference's);       */

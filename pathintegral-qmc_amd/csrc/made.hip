@@ -1,0 +1,620 @@
+// Learned slice proposal: a masked autoregressive network (MADE, Germain et
+// al. 2015) over one Trotter slice, trained on slices the plain sweep emits,
+// used as an independence Metropolis-Hastings proposal.
+//
+// Network (natural order): N -> H1 -> H2 -> N logits, ReLU hidden layers.
+// Degrees: input d has degree d+1; hidden unit k has m(k) = 1 + k mod (N-1);
+// masks  M1[k][d] = m1(k) >= d+1,  M2[k'][k] = m2(k') >= m1(k),
+//        M3[d][k] = d+1 > m2(k)   so logit d depends on spins < d only.
+// Logit l_d gives p(s_d = -1 | s_<d) = sigmoid(l_d); log q(s) = -sum_d
+// [softplus(l_d) - b_d l_d] with b_d = (s_d == -1).
+//
+// Arithmetic: BF16 weights and activations, FP32 accumulation on MFMA
+// (v_mfma_f32_16x16x32_bf16), FP32 master weights and Adam moments.  Every
+// GEMM is C[M][N] = A[M][K] . B[N][K]^T with both operands K-contiguous.
+//
+// Exactness of the proposal density.  The sampler computes logit d with the
+// same full forward pass (same kernels, same shapes, same summation order) on
+// the partially sampled slice; the weights that connect logit d to spins >= d
+// are exact zeros, so logit d is bit-identical to the one a forward pass over
+// the finished slice produces.  log q(new) and log q(old) therefore come from
+// one function q, the density the proposal was drawn from.
+#include <math.h>
+
+#include <algorithm>
+
+#include "piqmc_internal.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+__device__ __forceinline__ uint16_t f2bf_bits(float f)
+{
+    // round to nearest even (finite inputs)
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// ------------------------------------------------------------------ GEMM
+// C = A . B^T (+ bias[n]) (ReLU); A [M][K], B [N][K] bf16 (uint16 bits), row
+// strides lda/ldb; outputs fp32 Cf [M][ldc] and/or bf16 Cb [M][ldc].
+// Workgroup 256 threads = 4 wavefronts in 2x2; tile 64x64, K-step 32; each
+// wavefront owns a 32x32 sub-tile = 2x2 MFMA 16x16x32 blocks.
+constexpr int GT = 64, GK = 32, GKP = GK + 8;  // LDS row pitch (bf16) avoids bank conflicts
+
+__global__ __launch_bounds__(256) void gemm_nt_bf16(const uint16_t *A, int lda, const uint16_t *B, int ldb,
+                                                    int M, int N, int K, const float *bias, int relu,
+                                                    float *Cf, uint16_t *Cb, int ldc)
+{
+    __shared__ __align__(16) uint16_t As[GT][GKP];
+    __shared__ __align__(16) uint16_t Bs[GT][GKP];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // each thread stages 8 consecutive bf16 of one row of each tile
+    const int lr = tid >> 2, lk = (tid & 3) * 8;
+    for (int k0 = 0; k0 < K; k0 += GK) {
+        {
+            const int gm = m0 + lr, gk = k0 + lk;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (gm < M) {
+                if (gk + 8 <= K && ((lda & 7) == 0)) {
+                    v = *reinterpret_cast<const uint4 *>(A + (size_t)gm * lda + gk);
+                } else {
+                    uint16_t t[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) t[j] = (gk + j < K) ? A[(size_t)gm * lda + gk + j] : 0;
+                    v = *reinterpret_cast<uint4 *>(t);
+                }
+            }
+            *reinterpret_cast<uint4 *>(&As[lr][lk]) = v;
+            const int gn = n0 + lr;
+            uint4 w = make_uint4(0, 0, 0, 0);
+            if (gn < N) {
+                if (gk + 8 <= K && ((ldb & 7) == 0)) {
+                    w = *reinterpret_cast<const uint4 *>(B + (size_t)gn * ldb + gk);
+                } else {
+                    uint16_t t[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) t[j] = (gk + j < K) ? B[(size_t)gn * ldb + gk + j] : 0;
+                    w = *reinterpret_cast<uint4 *>(t);
+                }
+            }
+            *reinterpret_cast<uint4 *>(&Bs[lr][lk]) = w;
+        }
+        __syncthreads();
+        // fragments: lane holds row (lane&15), k = (lane>>4)*8 .. +7
+        bf16x8 af[2], bfr[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            uint4 va = *reinterpret_cast<const uint4 *>(&As[wm + t * 16 + (lane & 15)][(lane >> 4) * 8]);
+            uint4 vb = *reinterpret_cast<const uint4 *>(&Bs[wn + t * 16 + (lane & 15)][(lane >> 4) * 8]);
+            af[t] = *reinterpret_cast<bf16x8 *>(&va);
+            bfr[t] = *reinterpret_cast<bf16x8 *>(&vb);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        __syncthreads();
+    }
+    // accumulator layout: lane holds column (lane&15), rows (lane>>4)*4 + j
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int col = n0 + wn + b * 16 + (lane & 15);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = m0 + wm + a * 16 + (lane >> 4) * 4 + j;
+                if (row < M && col < N) {
+                    float v = acc[a][b][j];
+                    if (bias) v += bias[col];
+                    if (relu) v = v > 0.f ? v : 0.f;
+                    if (Cf) Cf[(size_t)row * ldc + col] = v;
+                    if (Cb) Cb[(size_t)row * ldc + col] = f2bf_bits(v);
+                }
+            }
+        }
+}
+
+void gemm(const uint16_t *A, int lda, const uint16_t *B, int ldb, int M, int N, int K, const float *bias,
+          int relu, float *Cf, uint16_t *Cb, int ldc, hipStream_t st)
+{
+    dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT);
+    gemm_nt_bf16<<<grid, 256, 0, st>>>(A, lda, B, ldb, M, N, K, bias, relu, Cf, Cb, ldc);
+}
+
+// ------------------------------------------------------------------ elementwise
+__global__ void spins_to_bf16(const int8_t *s, size_t n, uint16_t *x)
+{
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        x[t] = s[t] < 0 ? 0xBF80u : 0x3F80u;  // -1.0 / +1.0 in bf16
+}
+
+// transpose bf16 [R][C] -> [C][R]
+__global__ void transpose_bf16(const uint16_t *in, int R, int C, uint16_t *out)
+{
+    __shared__ uint16_t tile[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    for (int y = threadIdx.y; y < 32; y += blockDim.y) {
+        const int r = by + y, c = bx + threadIdx.x;
+        tile[y][threadIdx.x] = (r < R && c < C) ? in[(size_t)r * C + c] : 0;
+    }
+    __syncthreads();
+    for (int y = threadIdx.y; y < 32; y += blockDim.y) {
+        const int c = bx + y, r = by + threadIdx.x;
+        if (c < C && r < R) out[(size_t)c * R + r] = tile[threadIdx.x][y];
+    }
+}
+
+// Output layer loss: per sample nll = sum_d softplus(l_d) - b_d l_d; grad
+// dL/dl (scaled by gscale) to bf16 dlog [B][N]; per-sample nll to out.
+__global__ void output_nll(const float *logits, const int8_t *s, int B, int N, float gscale, uint16_t *dlog,
+                           double *nll, double sign)
+{
+    const int b = blockIdx.x;
+    double acc = 0.0;
+    for (int d = threadIdx.x; d < N; d += blockDim.x) {
+        const float l = logits[(size_t)b * N + d];
+        const float bit = s[(size_t)b * N + d] < 0 ? 1.f : 0.f;
+        const double sp = l > 0 ? (double)l + log1p(exp(-(double)l)) : log1p(exp((double)l));
+        acc += sp - (double)bit * (double)l;
+        if (dlog) {
+            const float sg = 1.f / (1.f + expf(-l));
+            dlog[(size_t)b * N + d] = f2bf_bits((sg - bit) * gscale);
+        }
+    }
+    __shared__ double red[16];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) t += red[w];
+        nll[b] = sign * t;
+    }
+}
+
+__global__ void fill_i8(int8_t *p, size_t n, int8_t v)
+{
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        p[t] = v;
+}
+
+// dH (bf16, in place) *= (H > 0): ReLU backward against the stored activation.
+__global__ void relu_back(uint16_t *dh, const uint16_t *h, size_t n)
+{
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        if (!(bf2f(h[t]) > 0.f)) dh[t] = 0;
+}
+
+// column sums of bf16 [B][C] -> fp32 [C]
+__global__ void col_sum_bf16(const uint16_t *g, int B, int C, float *out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    float s = 0.f;
+    for (int b = 0; b < B; ++b) s += bf2f(g[(size_t)b * C + c]);
+    out[c] = s;
+}
+
+// Adam step on fp32 master weights (masked), refresh bf16 W [O][I] and W^T [I][O].
+__global__ void adam_w(float *w, float *m, float *v, const float *g, const uint8_t *mask, int O, int I,
+                       float lr, float b1, float b2, float eps, float c1, float c2, uint16_t *wb, uint16_t *wt)
+{
+    const size_t n = (size_t)O * I;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        float wv = 0.f;
+        if (mask[t]) {
+            const float gr = g[t];
+            const float mv = b1 * m[t] + (1.f - b1) * gr;
+            const float vv = b2 * v[t] + (1.f - b2) * gr * gr;
+            m[t] = mv;
+            v[t] = vv;
+            wv = w[t] - lr * (mv / c1) / (sqrtf(vv / c2) + eps);
+        }
+        w[t] = wv;
+        const uint16_t bb = f2bf_bits(wv);
+        wb[t] = bb;
+        const int o = (int)(t / I), i = (int)(t % I);
+        wt[(size_t)i * O + o] = bb;
+    }
+}
+
+__global__ void adam_b(float *b, float *m, float *v, const float *g, int n, float lr, float b1, float b2,
+                       float eps, float c1, float c2)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float gr = g[t];
+    const float mv = b1 * m[t] + (1.f - b1) * gr;
+    const float vv = b2 * v[t] + (1.f - b2) * gr * gr;
+    m[t] = mv;
+    v[t] = vv;
+    b[t] -= lr * (mv / c1) / (sqrtf(vv / c2) + eps);
+}
+
+// Sampling step d: draw s_d for every row from sigmoid(logit d).
+__global__ void sample_step(const float *logits, int B, int N, int d, uint32_t k0, uint32_t k1, uint32_t round,
+                            int8_t *s, uint16_t *x)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const float l = logits[(size_t)b * N + d];
+    uint32_t out[4];
+    philox4x32_10(round, (uint32_t)b, (uint32_t)d, 0x5A4D0000u, k0, k1, out);
+    const double u = ((double)out[0] + 0.5) * 2.3283064365386963e-10;
+    const double p = 1.0 / (1.0 + exp(-(double)l));  // p(s_d = -1)
+    const bool neg = u < p;
+    s[(size_t)b * N + d] = neg ? -1 : 1;
+    x[(size_t)b * N + d] = neg ? 0xBF80u : 0x3F80u;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ MADE state
+struct piqmc_made {
+    int N = 0, H1 = 0, H2 = 0, cap = 0;  // cap = max batch rows of the buffers
+    int layers_in[3], layers_out[3];
+    float *w[3] = {}, *b[3] = {}, *mw[3] = {}, *vw[3] = {}, *mb[3] = {}, *vb[3] = {};
+    uint16_t *wb[3] = {}, *wt[3] = {};
+    uint8_t *mask[3] = {};
+    float *gw[3] = {}, *gb[3] = {};
+    // activations [cap][*]
+    uint16_t *x = nullptr, *h1 = nullptr, *h2 = nullptr;
+    float *logits = nullptr;
+    uint16_t *dlog = nullptr, *dh2 = nullptr, *dh1 = nullptr;
+    uint16_t *tA = nullptr, *tB = nullptr;  // transposes for weight gradients
+    double *nll = nullptr;
+    int8_t *s = nullptr;
+    int step = 0;
+};
+
+namespace {
+int made_alloc_batch(piqmc_made *m, int B)
+{
+    if (B <= m->cap) return PIQMC_OK;
+    const int N = m->N, H1 = m->H1, H2 = m->H2;
+    void **bufs[] = {(void **)&m->x, (void **)&m->h1, (void **)&m->h2, (void **)&m->logits, (void **)&m->dlog,
+                     (void **)&m->dh2, (void **)&m->dh1, (void **)&m->tA, (void **)&m->tB, (void **)&m->nll,
+                     (void **)&m->s};
+    for (auto p : bufs)
+        if (*p) (void)hipFree(*p), *p = nullptr;
+    const int Hm = std::max(std::max(H1, H2), N);
+    PIQMC_HIP(hipMalloc(&m->x, 2 * (size_t)B * N));
+    PIQMC_HIP(hipMalloc(&m->h1, 2 * (size_t)B * H1));
+    PIQMC_HIP(hipMalloc(&m->h2, 2 * (size_t)B * H2));
+    PIQMC_HIP(hipMalloc(&m->logits, 4 * (size_t)B * N));
+    PIQMC_HIP(hipMalloc(&m->dlog, 2 * (size_t)B * N));
+    PIQMC_HIP(hipMalloc(&m->dh2, 2 * (size_t)B * H2));
+    PIQMC_HIP(hipMalloc(&m->dh1, 2 * (size_t)B * H1));
+    PIQMC_HIP(hipMalloc(&m->tA, 2 * (size_t)B * Hm));
+    PIQMC_HIP(hipMalloc(&m->tB, 2 * (size_t)B * Hm));
+    PIQMC_HIP(hipMalloc(&m->nll, 8 * (size_t)B));
+    PIQMC_HIP(hipMalloc(&m->s, (size_t)B * N));
+    m->cap = B;
+    return PIQMC_OK;
+}
+
+int grid1(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 65535); }
+
+void transpose(const uint16_t *in, int R, int C, uint16_t *out, hipStream_t st)
+{
+    dim3 g((C + 31) / 32, (R + 31) / 32), blk(32, 8);
+    transpose_bf16<<<g, blk, 0, st>>>(in, R, C, out);
+}
+
+// forward over m->x (bf16 spins) rows [0, B): h1, h2 (bf16), logits (fp32)
+void forward(piqmc_made *m, int B, hipStream_t st)
+{
+    gemm(m->x, m->N, m->wb[0], m->N, B, m->H1, m->N, m->b[0], 1, nullptr, m->h1, m->H1, st);
+    gemm(m->h1, m->H1, m->wb[1], m->H1, B, m->H2, m->H1, m->b[1], 1, nullptr, m->h2, m->H2, st);
+    gemm(m->h2, m->H2, m->wb[2], m->H2, B, m->N, m->H2, m->b[2], 0, m->logits, nullptr, m->N, st);
+}
+}  // namespace
+
+extern "C" int piqmc_made_create(int N, int H1, int H2, uint64_t seed, piqmc_made **out)
+{
+    if (!out || N < 2 || H1 < 1 || H2 < 1) return piqmc::set_error(PIQMC_EINVAL, "made_create: bad sizes");
+    auto *m = new piqmc_made();
+    m->N = N; m->H1 = H1; m->H2 = H2;
+    const int ins[3] = {N, H1, H2}, outs[3] = {H1, H2, N};
+    // degrees and masks
+    std::vector<int> m1(H1), m2(H2);
+    for (int k = 0; k < H1; ++k) m1[k] = 1 + k % (N - 1);
+    for (int k = 0; k < H2; ++k) m2[k] = 1 + k % (N - 1);
+    // deterministic init: U(-1/sqrt(fan_in), 1/sqrt(fan_in)) from a 64-bit LCG
+    uint64_t rs = seed * 6364136223846793005ull + 1442695040888963407ull;
+    auto uni = [&]() {
+        rs = rs * 6364136223846793005ull + 1442695040888963407ull;
+        return ((rs >> 11) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0;
+    };
+    for (int l = 0; l < 3; ++l) {
+        const int O = outs[l], I = ins[l];
+        m->layers_in[l] = I; m->layers_out[l] = O;
+        std::vector<uint8_t> mk((size_t)O * I);
+        for (int o = 0; o < O; ++o)
+            for (int i = 0; i < I; ++i) {
+                bool on;
+                if (l == 0) on = m1[o] >= i + 1;
+                else if (l == 1) on = m2[o] >= m1[i];
+                else on = o + 1 > m2[i];
+                mk[(size_t)o * I + i] = on;
+            }
+        std::vector<float> wv((size_t)O * I);
+        const double sc = 1.0 / sqrt((double)I);
+        for (size_t t = 0; t < wv.size(); ++t) wv[t] = mk[t] ? (float)(uni() * sc) : 0.f;
+        std::vector<float> zero_o(O, 0.f);
+        size_t nw = (size_t)O * I;
+        hipError_t e = hipSuccess;
+        auto A = [&](void **p, size_t bytes) { if (e == hipSuccess) e = hipMalloc(p, bytes); };
+        A((void **)&m->w[l], 4 * nw); A((void **)&m->mw[l], 4 * nw); A((void **)&m->vw[l], 4 * nw);
+        A((void **)&m->gw[l], 4 * nw); A((void **)&m->wb[l], 2 * nw); A((void **)&m->wt[l], 2 * nw);
+        A((void **)&m->mask[l], nw); A((void **)&m->b[l], 4 * O); A((void **)&m->mb[l], 4 * O);
+        A((void **)&m->vb[l], 4 * O); A((void **)&m->gb[l], 4 * O);
+        if (e == hipSuccess) e = hipMemcpy(m->w[l], wv.data(), 4 * nw, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(m->mask[l], mk.data(), nw, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(m->mw[l], 0, 4 * nw);
+        if (e == hipSuccess) e = hipMemset(m->vw[l], 0, 4 * nw);
+        if (e == hipSuccess) e = hipMemset(m->gw[l], 0, 4 * nw);
+        if (e == hipSuccess) e = hipMemset(m->b[l], 0, 4 * O);
+        if (e == hipSuccess) e = hipMemset(m->mb[l], 0, 4 * O);
+        if (e == hipSuccess) e = hipMemset(m->vb[l], 0, 4 * O);
+        if (e != hipSuccess) {
+            piqmc_made_destroy(m);
+            return piqmc::hip_fail(e, "made_create");
+        }
+        // bf16 copies: an Adam step with zero gradient and zero lr refreshes them
+        adam_w<<<grid1(nw), 256>>>(m->w[l], m->mw[l], m->vw[l], m->gw[l], m->mask[l], O, I, 0.f, 0.9f, 0.999f,
+                                   1e-8f, 1.f, 1.f, m->wb[l], m->wt[l]);
+    }
+    PIQMC_HIP(hipDeviceSynchronize());
+    *out = m;
+    return PIQMC_OK;
+}
+
+extern "C" int piqmc_made_destroy(piqmc_made *m)
+{
+    if (!m) return PIQMC_OK;
+    for (int l = 0; l < 3; ++l) {
+        void *ps[] = {m->w[l], m->b[l], m->mw[l], m->vw[l], m->mb[l], m->vb[l], m->wb[l], m->wt[l], m->mask[l],
+                      m->gw[l], m->gb[l]};
+        for (void *p : ps)
+            if (p) (void)hipFree(p);
+    }
+    void *bs[] = {m->x, m->h1, m->h2, m->logits, m->dlog, m->dh2, m->dh1, m->tA, m->tB, m->nll, m->s};
+    for (void *p : bs)
+        if (p) (void)hipFree(p);
+    delete m;
+    return PIQMC_OK;
+}
+
+// Parameters as fp32 host arrays: w0 [H1][N], b0 [H1], w1 [H2][H1], b1 [H2], w2 [N][H2], b2 [N].
+extern "C" int piqmc_made_get_params(const piqmc_made *m, float *w0, float *b0, float *w1, float *b1, float *w2,
+                                     float *b2)
+{
+    if (!m) return piqmc::set_error(PIQMC_EINVAL, "null made");
+    float *ws[3] = {w0, w1, w2}, *bs[3] = {b0, b1, b2};
+    for (int l = 0; l < 3; ++l) {
+        const size_t nw = (size_t)m->layers_out[l] * m->layers_in[l];
+        if (ws[l]) PIQMC_HIP(hipMemcpy(ws[l], m->w[l], 4 * nw, hipMemcpyDeviceToHost));
+        if (bs[l]) PIQMC_HIP(hipMemcpy(bs[l], m->b[l], 4 * (size_t)m->layers_out[l], hipMemcpyDeviceToHost));
+    }
+    return PIQMC_OK;
+}
+
+extern "C" int piqmc_made_get_grads(const piqmc_made *m, float *w0, float *b0, float *w1, float *b1, float *w2,
+                                    float *b2)
+{
+    if (!m) return piqmc::set_error(PIQMC_EINVAL, "null made");
+    float *ws[3] = {w0, w1, w2}, *bs[3] = {b0, b1, b2};
+    for (int l = 0; l < 3; ++l) {
+        const size_t nw = (size_t)m->layers_out[l] * m->layers_in[l];
+        if (ws[l]) PIQMC_HIP(hipMemcpy(ws[l], m->gw[l], 4 * nw, hipMemcpyDeviceToHost));
+        if (bs[l]) PIQMC_HIP(hipMemcpy(bs[l], m->gb[l], 4 * (size_t)m->layers_out[l], hipMemcpyDeviceToHost));
+    }
+    return PIQMC_OK;
+}
+
+// log q of B slices (int8 [B][N], device) -> logq_dev [B] (float64).
+extern "C" int piqmc_made_logprob(piqmc_made *m, int B, const int8_t *s_dev, double *logq_dev, void *stream)
+{
+    if (!m || B < 0 || (B && (!s_dev || !logq_dev))) return piqmc::set_error(PIQMC_EINVAL, "made_logprob: bad args");
+    if (!B) return PIQMC_OK;
+    int rc = made_alloc_batch(m, B);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    spins_to_bf16<<<grid1((size_t)B * m->N), 256, 0, st>>>(s_dev, (size_t)B * m->N, m->x);
+    forward(m, B, st);
+    output_nll<<<B, 256, 0, st>>>(m->logits, s_dev, B, m->N, 0.f, nullptr, m->nll, -1.0);
+    PIQMC_HIP(hipMemcpyAsync(logq_dev, m->nll, 8 * (size_t)B, hipMemcpyDeviceToDevice, st));
+    PIQMC_HIP(hipGetLastError());
+    return PIQMC_OK;
+}
+
+// Gradients of the mean NLL over B slices into the gradient buffers; returns
+// the mean NLL.  With lr > 0 also applies one Adam step.
+extern "C" int piqmc_made_train_step(piqmc_made *m, int B, const int8_t *s_dev, float lr, double *mean_nll,
+                                     void *stream)
+{
+    if (!m || B < 1 || !s_dev) return piqmc::set_error(PIQMC_EINVAL, "made_train_step: bad args");
+    int rc = made_alloc_batch(m, B);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const int N = m->N, H1 = m->H1, H2 = m->H2;
+    spins_to_bf16<<<grid1((size_t)B * N), 256, 0, st>>>(s_dev, (size_t)B * N, m->x);
+    forward(m, B, st);
+    output_nll<<<B, 256, 0, st>>>(m->logits, s_dev, B, N, 1.f / (float)B, m->dlog, m->nll, 1.0);
+    // layer 3: dW2 = dlog^T h2, db2 = colsum(dlog), dh2 = dlog W2
+    transpose(m->dlog, B, N, m->tA, st);   // [N][B]
+    transpose(m->h2, B, H2, m->tB, st);    // [H2][B]
+    gemm(m->tA, B, m->tB, B, N, H2, B, nullptr, 0, m->gw[2], nullptr, H2, st);
+    col_sum_bf16<<<(N + 255) / 256, 256, 0, st>>>(m->dlog, B, N, m->gb[2]);
+    gemm(m->dlog, N, m->wt[2], N, B, H2, N, nullptr, 0, nullptr, m->dh2, H2, st);
+    relu_back<<<grid1((size_t)B * H2), 256, 0, st>>>(m->dh2, m->h2, (size_t)B * H2);
+    // layer 2
+    transpose(m->dh2, B, H2, m->tA, st);   // [H2][B]
+    transpose(m->h1, B, H1, m->tB, st);    // [H1][B]
+    gemm(m->tA, B, m->tB, B, H2, H1, B, nullptr, 0, m->gw[1], nullptr, H1, st);
+    col_sum_bf16<<<(H2 + 255) / 256, 256, 0, st>>>(m->dh2, B, H2, m->gb[1]);
+    gemm(m->dh2, H2, m->wt[1], H2, B, H1, H2, nullptr, 0, nullptr, m->dh1, H1, st);
+    relu_back<<<grid1((size_t)B * H1), 256, 0, st>>>(m->dh1, m->h1, (size_t)B * H1);
+    // layer 1
+    transpose(m->dh1, B, H1, m->tA, st);   // [H1][B]
+    transpose(m->x, B, N, m->tB, st);      // [N][B]
+    gemm(m->tA, B, m->tB, B, H1, N, B, nullptr, 0, m->gw[0], nullptr, N, st);
+    col_sum_bf16<<<(H1 + 255) / 256, 256, 0, st>>>(m->dh1, B, H1, m->gb[0]);
+    if (lr > 0.f) {
+        m->step++;
+        const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+        const float c1 = 1.f - powf(b1, (float)m->step), c2 = 1.f - powf(b2, (float)m->step);
+        for (int l = 0; l < 3; ++l) {
+            const int O = m->layers_out[l], I = m->layers_in[l];
+            adam_w<<<grid1((size_t)O * I), 256, 0, st>>>(m->w[l], m->mw[l], m->vw[l], m->gw[l], m->mask[l], O, I, lr,
+                                                         b1, b2, eps, c1, c2, m->wb[l], m->wt[l]);
+            adam_b<<<(O + 255) / 256, 256, 0, st>>>(m->b[l], m->mb[l], m->vb[l], m->gb[l], O, lr, b1, b2, eps, c1, c2);
+        }
+    }
+    PIQMC_HIP(hipGetLastError());
+    if (mean_nll) {
+        std::vector<double> h(B);
+        PIQMC_HIP(hipMemcpyAsync(h.data(), m->nll, 8 * (size_t)B, hipMemcpyDeviceToHost, st));
+        PIQMC_HIP(hipStreamSynchronize(st));
+        double t = 0;
+        for (double v : h) t += v;
+        *mean_nll = t / B;
+    }
+    return PIQMC_OK;
+}
+
+// Draw B slices from q (autoregressive, N forward passes), int8 [B][N] to
+// s_out_dev, log q of each draw to logq_dev.  Draw d of row b uses
+// philox(key=seed, ctr={round, b, d, 0x5A4D0000})[0].
+extern "C" int piqmc_made_sample(piqmc_made *m, int B, uint64_t seed, uint32_t round, int8_t *s_out_dev,
+                                 double *logq_dev, void *stream)
+{
+    if (!m || B < 1 || !s_out_dev) return piqmc::set_error(PIQMC_EINVAL, "made_sample: bad args");
+    int rc = made_alloc_batch(m, B);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const int N = m->N;
+    fill_i8<<<grid1((size_t)B * N), 256, 0, st>>>(m->s, (size_t)B * N, 1);  // placeholder spins beyond the prefix
+    spins_to_bf16<<<grid1((size_t)B * N), 256, 0, st>>>(m->s, (size_t)B * N, m->x);
+    for (int d = 0; d < N; ++d) {
+        forward(m, B, st);
+        sample_step<<<(B + 255) / 256, 256, 0, st>>>(m->logits, B, N, d, (uint32_t)seed, (uint32_t)(seed >> 32),
+                                                     round, m->s, m->x);
+    }
+    forward(m, B, st);
+    output_nll<<<B, 256, 0, st>>>(m->logits, m->s, B, N, 0.f, nullptr, m->nll, -1.0);
+    PIQMC_HIP(hipMemcpyAsync(s_out_dev, m->s, (size_t)B * N, hipMemcpyDeviceToDevice, st));
+    if (logq_dev) PIQMC_HIP(hipMemcpyAsync(logq_dev, m->nll, 8 * (size_t)B, hipMemcpyDeviceToDevice, st));
+    PIQMC_HIP(hipGetLastError());
+    return PIQMC_OK;
+}
+
+namespace {
+// old slice of replica r -> rows [R][N]
+__global__ void gather_slices(const int8_t *confs, int R, int P, int N, const int32_t *slice, int8_t *out)
+{
+    const size_t tot = (size_t)R * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(t / N), i = (int)(t % N);
+        out[t] = confs[((size_t)r * P + slice[r]) * N + i];
+    }
+}
+
+// One workgroup per replica: action change of replacing slice k by the
+// proposal, Metropolis-Hastings test with the proposal densities, write-back.
+__global__ void mh_kernel(int8_t *confs, int R, int P, int N, int Z, const int2 *rows, const int8_t *newsl,
+                          const double *lq_new, const double *lq_old, const int32_t *slice, float jperp,
+                          float temp, uint32_t k0, uint32_t k1, uint32_t round, int32_t *accepted)
+{
+    const int r = blockIdx.x;
+    const int k = slice[r];
+    const int8_t *old = confs + ((size_t)r * P + k) * N;
+    const int8_t *nw = newsl + (size_t)r * N;
+    const int8_t *lft = confs + ((size_t)r * P + (k == 0 ? P - 1 : k - 1)) * N;
+    const int8_t *rgt = confs + ((size_t)r * P + (k == P - 1 ? 0 : k + 1)) * N;
+    double dE = 0.0, dT = 0.0;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const double so = old[i], sn = nw[i];
+        for (int e = 0; e < Z; ++e) {
+            const int2 rw = rows[(size_t)i * Z + e];
+            const double J = (double)__int_as_float(rw.y);
+            if (rw.x == i) dE -= J * (sn - so);
+            else dE -= 0.5 * J * (sn * (double)nw[rw.x] - so * (double)old[rw.x]);
+        }
+        dT -= (double)jperp * (sn - so) * ((double)lft[i] + (double)rgt[i]);
+    }
+    __shared__ double red[2][16];
+    for (int o = 32; o > 0; o >>= 1) {
+        dE += __shfl_xor(dE, o, 64);
+        dT += __shfl_xor(dT, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = dE;
+        red[1][threadIdx.x >> 6] = dT;
+    }
+    __syncthreads();
+    __shared__ int acc_flag;
+    if (threadIdx.x == 0) {
+        double a = 0, b = 0;
+        for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) a += red[0][w], b += red[1][w];
+        const double logA = -(a + b) / (double)temp + lq_old[r] - lq_new[r];
+        uint32_t out[4];
+        philox4x32_10(round, (uint32_t)r, 0u, 0x4D480000u, k0, k1, out);
+        const double u = ((double)out[0] + 0.5) * 2.3283064365386963e-10;
+        acc_flag = (logA >= 0.0 || log(u) < logA) ? 1 : 0;
+        accepted[r] = acc_flag;
+    }
+    __syncthreads();
+    if (acc_flag) {
+        int8_t *dst = confs + ((size_t)r * P + k) * N;
+        for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = nw[i];
+    }
+}
+}  // namespace
+
+extern "C" int piqmc_made_mh(piqmc_made *m, const piqmc_problem *pb, int R, int P, int8_t *confs,
+                             const int8_t *newsl, const double *lq_new, const int32_t *slice, float jperp,
+                             float temp, uint64_t seed, uint32_t round, int32_t *accepted, void *stream)
+{
+    if (!m || !pb || R < 1 || P < 1 || !confs || !newsl || !lq_new || !slice || !accepted)
+        return piqmc::set_error(PIQMC_EINVAL, "made_mh: bad args");
+    if (pb->N != m->N) return piqmc::set_error(PIQMC_EINVAL, "made_mh: network and problem sizes differ");
+    hipStream_t st = (hipStream_t)stream;
+    int8_t *old = nullptr;
+    double *lq_old = nullptr;
+    PIQMC_HIP(hipMallocAsync((void **)&old, (size_t)R * m->N, st));
+    PIQMC_HIP(hipMallocAsync((void **)&lq_old, 8 * (size_t)R, st));
+    gather_slices<<<grid1((size_t)R * m->N), 256, 0, st>>>(confs, R, P, m->N, slice, old);
+    int rc = piqmc_made_logprob(m, R, old, lq_old, stream);
+    if (rc) return rc;
+    mh_kernel<<<R, 256, 0, st>>>(confs, R, P, m->N, pb->Z, pb->d_rows, newsl, lq_new, lq_old, slice, jperp, temp,
+                                 (uint32_t)seed, (uint32_t)(seed >> 32), round, accepted);
+    PIQMC_HIP(hipFreeAsync(old, st));
+    PIQMC_HIP(hipFreeAsync(lq_old, st));
+    PIQMC_HIP(hipGetLastError());
+    return PIQMC_OK;
+}
+
+// Plain GEMM entry for tests: C[M][N] = A[M][K] . B[N][K]^T (bf16 in, fp32 out).
+extern "C" int piqmc_gemm_bf16_nt(const uint16_t *A, const uint16_t *B, int M, int N, int K, float *C, void *stream)
+{
+    if (!A || !B || !C || M < 1 || N < 1 || K < 1) return piqmc::set_error(PIQMC_EINVAL, "gemm: bad args");
+    gemm(A, K, B, K, M, N, K, nullptr, 0, C, nullptr, N, (hipStream_t)stream);
+    PIQMC_HIP(hipGetLastError());
+    return PIQMC_OK;
+}

@@ -43,6 +43,15 @@ _SIGS = {
     "piqmc_slice_energies": ([_vp, _i, _vp, _vp, _vp], _i),
     "piqmc_classical_energy": ([_i, _i64, _vp, _vp, _vp, _i, _vp, _vp], _i),
     "piqmc_generate_neighbors": ([_i, _i, _i64, _vp, _vp, _vp, _i, _vp], _i),
+    "piqmc_made_create": ([_i, _i, _i, _u64, C.POINTER(_vp)], _i),
+    "piqmc_made_destroy": ([_vp], _i),
+    "piqmc_made_get_params": ([_vp] * 7, _i),
+    "piqmc_made_get_grads": ([_vp] * 7, _i),
+    "piqmc_made_logprob": ([_vp, _i, _vp, _vp, _vp], _i),
+    "piqmc_made_train_step": ([_vp, _i, _vp, _f, _vp, _vp], _i),
+    "piqmc_made_sample": ([_vp, _i, _u64, _u32, _vp, _vp, _vp], _i),
+    "piqmc_made_mh": ([_vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _f, _f, _u64, _u32, _vp, _vp], _i),
+    "piqmc_gemm_bf16_nt": ([_vp, _vp, _i, _i, _i, _vp, _vp], _i),
 }
 for _name, (_args, _res) in _SIGS.items():
     _fn = getattr(lib, _name)
