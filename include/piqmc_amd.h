@@ -61,6 +61,10 @@ int piqmc_problem_info(const piqmc_problem *pb, int *nspins, int *maxnb,
 int piqmc_problem_set_coloring(piqmc_problem *pb, int ncolors, const int32_t *color_host);
 /* The coloring in use (color_host[nspins]). */
 int piqmc_problem_get_coloring(const piqmc_problem *pb, int32_t *color_host);
+/* Parallel-sweep kernel: 0 automatic (dense when color classes average fewer
+ * than 8 spins), 1 one parity word per thread, 2 one replica per wavefront
+ * with a lane per slice.  Every choice gives identical results. */
+int piqmc_problem_set_sweep_path(piqmc_problem *pb, int path);
 
 /* ------------------------------------------------------------------ */
 /* Random streams of the reference, for replay.                       */
@@ -128,6 +132,25 @@ int piqmc_qa_colored(const piqmc_problem *pb, int R, int P, int S, int M, float 
                      int block, int rpw, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Batched heterogeneous parallel sweeps (no reference counterpart: the */
+/* reference runs one replica per call).  Jobs of any mix of problems,  */
+/* slice counts, schedule lengths and replica counts run in ONE         */
+/* persistent launch that takes work items from a device counter;       */
+/* each job's result equals its own piqmc_qa_colored call.              */
+/* ------------------------------------------------------------------ */
+typedef struct piqmc_batch piqmc_batch;
+int piqmc_batch_create(piqmc_batch **out);
+int piqmc_batch_destroy(piqmc_batch *b);
+/* Same arguments and meaning as piqmc_qa_colored (sched is copied). */
+int piqmc_batch_add(piqmc_batch *b, const piqmc_problem *pb, int R, int P, int S, int M,
+                    float temp, const double *sched_host, int8_t *confs_dev, uint64_t seed,
+                    uint32_t replica_base, uint32_t sweep_base, double *etrace_dev);
+/* One launch over every job; block: threads per workgroup (0 = 256), rpw:
+ * replicas per work item for word-per-thread jobs (0 = automatic). */
+int piqmc_batch_run(piqmc_batch *b, int block, int rpw, void *stream);
+int piqmc_batch_info(const piqmc_batch *b, int *njobs, int *nitems, int *grid);
+
+/* ------------------------------------------------------------------ */
 /* Energies.                                                           */
 /* ------------------------------------------------------------------ */
 /* Classical energy of each of B int8 configurations [B][N] from the
@@ -160,9 +183,13 @@ int piqmc_made_logprob(piqmc_made *m, int B, const int8_t *s_dev, double *logq_d
  * (beta 0.9/0.999, eps 1e-8) when lr > 0; *mean_nll (host, may be NULL). */
 int piqmc_made_train_step(piqmc_made *m, int B, const int8_t *s_dev, float lr, double *mean_nll,
                           void *stream);
-/* B autoregressive draws -> int8 [B][N] and their log q (float64 [B]). */
+/* B autoregressive draws -> int8 [B][N] and their log q (float64 [B]).
+ * Draw d of row b uses philox(key=seed, ctr={round, b, d, 0x5A4D0000})[0].
+ * The N-step launch chain runs as a captured HIP graph (replayed per call)
+ * unless piqmc_made_set_graph(m, 0) selects eager launches. */
 int piqmc_made_sample(piqmc_made *m, int B, uint64_t seed, uint32_t round, int8_t *s_out_dev,
                       double *logq_dev, void *stream);
+int piqmc_made_set_graph(piqmc_made *m, int use_graph);
 /* Independence Metropolis-Hastings slice move for R replicas of confs_dev
  * int8 [R][P][N]: replica r proposes new_dev[r] (log q in logq_new_dev[r])
  * for slice k = slice_dev[r]; accepted with min(1, exp(-dS/T) q(old)/q(new)),

@@ -110,13 +110,11 @@ __device__ __forceinline__ WT ge_pick(int k, WT c0, WT c1, WT c2)
     }
 }
 
+// Replicas r0 .. r0+nrep-1 of one problem, whole schedule, by one workgroup.
 template <typename WT, bool PMJ>
-__global__ void colored_kernel(ColArgs<WT> a)
+__device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned char *smem)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
     const int N = a.N, W = a.W, tid = threadIdx.x, bd = blockDim.x;
-    const int r0 = blockIdx.x * a.rpw;
-    const int nrep = min(a.rpw, a.R - r0);
     WT *st = reinterpret_cast<WT *>(smem);                            // [nrep][2][N]
     uint32_t *tthr = reinterpret_cast<uint32_t *>(st + (size_t)2 * N * a.rpw);  // [kClassStride]
     uint32_t *tz = tthr + kClassStride;                               // [8]
@@ -317,6 +315,109 @@ __global__ void colored_kernel(ColArgs<WT> a)
     for (int q = tid; q < 2 * N * nrep; q += bd) gst[q] = st[q];
 }
 
+template <typename WT, bool PMJ>
+__global__ void colored_kernel(ColArgs<WT> a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int r0 = blockIdx.x * a.rpw;
+    colored_body<WT, PMJ>(a, r0, min(a.rpw, a.R - r0), smem);
+}
+
+// ------------------------------------------------------------------ dense path
+// Graphs whose color classes are tiny (dense couplings such as
+// Sherrington-Kirkpatrick need one color per spin) leave the word-per-thread
+// sweep with one busy thread per phase.  Here one wavefront owns one replica
+// and its lanes own the slices of the parity word being updated: lane m
+// evaluates the exact double local field of (spin i, slice 2m+tau) in row
+// order -- the CPU twin's arithmetic -- and the word's flips are gathered
+// with a ballot.  Stochastic lanes read their uniform from the same
+// bit-sliced Philox words as the word-per-thread path.
+template <typename WT>
+__device__ void dense_replica(const ColArgs<WT> &a, int r, WT *st)
+{
+    const int N = a.N, W = a.W, lane = threadIdx.x & 63;
+    const bool active = lane < W;
+    const WT one = WT(1);
+    const uint32_t rep = a.replica_base + (uint32_t)r;
+    WT *gst = a.packed + (size_t)r * 2 * N;
+    for (int q = lane; q < 2 * N; q += 64) st[q] = gst[q];
+    for (int s = 0; s < a.S; ++s) {
+        const float jp = a.jperp[s];
+        for (int m = 0; m < a.M; ++m) {
+            const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
+            for (int tau = 0; tau < 2; ++tau) {
+                WT *mine = st + tau * N;
+                const WT *other = st + (1 - tau) * N;
+                for (int q = 0; q < N; ++q) {  // colors in order; spins of one color are independent
+                    const int i = a.order[q];
+                    const WT S_ = mine[i], O = other[i];
+                    const int lb = lane < W ? lane : 0;
+                    // Trotter neighbours of slice 2lb+tau: slices 2lb+tau-1 and 2lb+tau+1
+                    const int ml = tau == 0 ? (lb == 0 ? W - 1 : lb - 1) : lb;
+                    const int mr = tau == 0 ? lb : (lb == W - 1 ? 0 : lb + 1);
+                    const double si = ((S_ >> lb) & one) ? -1.0 : 1.0;
+                    double h = 0.0;
+                    const int2 *row = a.rows + (size_t)i * a.Z;
+                    for (int e = 0; e < a.Z; ++e) {
+                        const int2 rw = row[e];
+                        const double J = (double)__int_as_float(rw.y);
+                        if (rw.x == i) h += J;
+                        else h += J * (((mine[rw.x] >> lb) & one) ? -1.0 : 1.0);
+                    }
+                    const double ts = (((O >> ml) & one) ? -1.0 : 1.0) + (((O >> mr) & one) ? -1.0 : 1.0);
+                    const double ed = -2.0 * si * (h + (double)jp * ts);
+                    unsigned int thr = 0;
+                    const int cls = piqmc_fixed_class(ed, (double)a.temp, &thr);
+                    bool fl = active && cls == 1;
+                    if (__ballot(active && cls == 2)) {
+                        constexpr int kPer = sizeof(WT) == 4 ? 4 : 2;
+                        uint32_t u = 0;
+                        for (int qd = 0; qd < 32 / kPer; ++qd) {
+                            uint32_t out[4];
+                            philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd, a.key0,
+                                          a.key1, out);
+#pragma unroll
+                            for (int t = 0; t < kPer; ++t)
+                                u |= (uint32_t)((rword<WT>(out, t) >> lb) & one) << (31 - (qd * kPer + t));
+                        }
+                        if (active && cls == 2) fl = u < thr;
+                    }
+                    const unsigned long long bal = __ballot(fl);
+                    if (lane == 0) mine[i] = S_ ^ (WT)bal;
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+        }
+        if (a.etrace) {
+            double acc = 0.0;
+            for (int i = lane; i < N; i += 64) {
+                const WT a0 = st[i], a1 = st[N + i];
+                const int2 *row = a.rows + (size_t)i * a.Z;
+                for (int e = 0; e < a.Z; ++e) {
+                    const int2 rw = row[e];
+                    const double J = (double)__int_as_float(rw.y);
+                    if (J == 0.0) continue;
+                    if (rw.x == i) acc -= J * (double)(a.P - 2 * (popc(a0) + popc(a1)));
+                    else acc -= 0.5 * J * (double)(a.P - 2 * (popc(a0 ^ st[rw.x]) + popc(a1 ^ st[N + rw.x])));
+                }
+            }
+            acc = wave_sum(acc);
+            if (lane == 0) a.etrace[(size_t)r * a.S + s] = acc / (double)a.P;
+        }
+    }
+    for (int q = lane; q < 2 * N; q += 64) gst[q] = st[q];
+}
+
+// One replica per wavefront, four wavefronts per workgroup, no barriers.
+template <typename WT>
+__global__ void dense_kernel(ColArgs<WT> a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int w = threadIdx.x >> 6, r = blockIdx.x * (blockDim.x >> 6) + w;
+    if (r >= a.R) return;
+    dense_replica<WT>(a, r, reinterpret_cast<WT *>(smem) + (size_t)w * 2 * a.N);
+}
+
 // int8 [R][P][N] -> packed [R][2][N] (bit m of parity tau = slice 2m+tau, set = -1)
 template <typename WT>
 __global__ void pack_parity_kernel(const int8_t *confs, WT *packed, int R, int P, int N)
@@ -355,43 +456,85 @@ int grid_for(size_t n, int block)
     return (int)(g > 65535 ? 65535 : (g < 1 ? 1 : g));
 }
 
+// Per-field-step tables for the +-1 path.  The decision class of a bit is a
+// function of (z, u_s, u_t) and, at fixed (z, u_t), monotone in the
+// unsatisfied-bond count u_s (ediff grows by 4 per unit), so each (z, u_t) is
+// summarised by two thresholds: accept for u_s >= amin, draw for
+// smin <= u_s < amin, reject below.  ztab packs amin/smin of u_t = 0,1,2 in
+// nibbles (amin, smin) at bits 0,4 / 8,12 / 16,20; thr holds the draw
+// thresholds.  Returns false if some class is not monotone (then the per-bit
+// path is used).
+bool pmj_tables(const piqmc_problem *pb, int S, const std::vector<float> &jp, float temp,
+                std::vector<uint32_t> &thr, std::vector<uint32_t> &ztab)
+{
+    thr.assign((size_t)S * kClassStride, 0);
+    ztab.assign((size_t)S * 8, 0);
+    for (int s = 0; s < S; ++s)
+        for (int z = 0; z <= pb->zmax; ++z)
+            for (int ut = 0; ut < 3; ++ut) {
+                int cls[9];
+                for (int us = 0; us <= z; ++us) {
+                    // -2 s_i (h + Jp*ts) with s_i*h = z - 2u_s, s_i*ts = 2 - 2u_t (exact)
+                    const double ed = -2.0 * ((double)(z - 2 * us) + (double)jp[s] * (double)(2 - 2 * ut));
+                    unsigned int t = 0;
+                    cls[us] = piqmc_fixed_class(ed, (double)temp, &t);
+                    if (cls[us] == 2) thr[(size_t)s * kClassStride + (z * 8 + us) * 3 + ut] = t;
+                }
+                int amin = z + 1, smin = z + 1;
+                for (int us = z; us >= 0 && cls[us] == 1; --us) amin = us;
+                for (int us = amin - 1; us >= 0 && cls[us] == 2; --us) smin = us;
+                if (smin > amin) smin = amin;
+                for (int us = 0; us < smin; ++us)
+                    if (cls[us] != 0) return false;
+                ztab[(size_t)s * 8 + z] |= ((uint32_t)amin | ((uint32_t)smin << 4)) << (8 * ut);
+            }
+    return true;
+}
+
+bool use_dense(const piqmc_problem *pb)
+{
+    return pb->sweep_path == 2 || (pb->sweep_path == 0 && pb->N < 8 * pb->ncolors);
+}
+
 template <typename WT>
 int run_colored(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
                 const std::vector<float> &jp, int8_t *confs, uint64_t seed, uint32_t rbase,
                 uint32_t sbase, double *etrace, int block, int rpw, hipStream_t st)
 {
     const int N = pb->N;
-    bool pmj = pb->pmj != 0;
-    // Per-field-step tables for the +-1 path.  The decision class of a bit is
-    // a function of (z, u_s, u_t) and, at fixed (z, u_t), monotone in the
-    // unsatisfied-bond count u_s (ediff grows by 4 per unit), so each (z, u_t)
-    // is summarised by two thresholds: accept for u_s >= amin, draw for
-    // smin <= u_s < amin, reject below.  ztab packs amin/smin of u_t = 0,1,2
-    // in nibbles (amin, smin) at bits 0,4 / 8,12 / 16,20.
-    std::vector<uint32_t> thr, ztab;
-    if (pmj) {
-        thr.assign((size_t)S * kClassStride, 0);
-        ztab.assign((size_t)S * 8, 0);
-        for (int s = 0; s < S && pmj; ++s)
-            for (int z = 0; z <= pb->zmax && pmj; ++z)
-                for (int ut = 0; ut < 3 && pmj; ++ut) {
-                    int cls[9];
-                    for (int us = 0; us <= z; ++us) {
-                        // -2 s_i (h + Jp*ts) with s_i*h = z - 2u_s, s_i*ts = 2 - 2u_t (exact)
-                        const double ed = -2.0 * ((double)(z - 2 * us) + (double)jp[s] * (double)(2 - 2 * ut));
-                        unsigned int t = 0;
-                        cls[us] = piqmc_fixed_class(ed, (double)temp, &t);
-                        if (cls[us] == 2) thr[(size_t)s * kClassStride + (z * 8 + us) * 3 + ut] = t;
-                    }
-                    int amin = z + 1, smin = z + 1;
-                    for (int us = z; us >= 0 && cls[us] == 1; --us) amin = us;
-                    for (int us = amin - 1; us >= 0 && cls[us] == 2; --us) smin = us;
-                    if (smin > amin) smin = amin;
-                    for (int us = 0; us < smin; ++us)
-                        if (cls[us] != 0) pmj = false;  // not monotone: use the per-bit path
-                    ztab[(size_t)s * 8 + z] |= ((uint32_t)amin | ((uint32_t)smin << 4)) << (8 * ut);
-                }
+    // dense path when color classes average fewer than 8 spins (or forced)
+    if (use_dense(pb)) {
+        if (block <= 0) block = 256;
+        if (block > 1024 || (block & 63)) return piqmc::set_error(PIQMC_EINVAL, "block must be a multiple of 64, <= 1024");
+        const int wpb = block / 64;
+        const size_t lds = sizeof(WT) * 2 * (size_t)N * wpb;
+        if (lds > 160 * 1024) return piqmc::set_error(PIQMC_EINVAL, "dense sweep: state exceeds LDS");
+        float *d_jp = nullptr;
+        WT *d_packed = nullptr;
+        PIQMC_HIP(hipMallocAsync((void **)&d_jp, sizeof(float) * (S ? S : 1), st));
+        if (S) PIQMC_HIP(hipMemcpyAsync(d_jp, jp.data(), sizeof(float) * S, hipMemcpyHostToDevice, st));
+        PIQMC_HIP(hipMallocAsync((void **)&d_packed, sizeof(WT) * 2 * (size_t)N * R, st));
+        pack_parity_kernel<WT><<<grid_for((size_t)R * 2 * N, 256), 256, 0, st>>>(confs, d_packed, R, P, N);
+        ColArgs<WT> a = {};
+        a.N = N; a.Z = pb->Z; a.P = P; a.W = P / 2; a.S = S; a.M = M; a.R = R;
+        a.ncolors = pb->ncolors; a.temp = temp;
+        a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
+        a.replica_base = rbase; a.sweep_base = sbase;
+        a.jperp = d_jp; a.rows = pb->d_rows; a.order = pb->d_order;
+        a.packed = d_packed; a.etrace = etrace;
+        piqmc::timing_begin(st);
+        (void)hipFuncSetAttribute((const void *)dense_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        dense_kernel<WT><<<(R + wpb - 1) / wpb, block, lds, st>>>(a);
+        piqmc::timing_end(st);
+        PIQMC_HIP(hipGetLastError());
+        unpack_parity_kernel<WT><<<grid_for((size_t)R * P * N, 256), 256, 0, st>>>(d_packed, confs, R, P, N);
+        PIQMC_HIP(hipFreeAsync(d_packed, st));
+        PIQMC_HIP(hipFreeAsync(d_jp, st));
+        PIQMC_HIP(hipGetLastError());
+        return PIQMC_OK;
     }
+    std::vector<uint32_t> thr, ztab;
+    const bool pmj = pb->pmj && pmj_tables(pb, S, jp, temp, thr, ztab);
     if (rpw <= 0) rpw = R >= 2048 ? 4 : (R >= 1024 ? 2 : 1);
     if (rpw > 64) return piqmc::set_error(PIQMC_EINVAL, "replicas per workgroup must be <= 64");
     const size_t lds = sizeof(WT) * 2 * (size_t)N * rpw + 4 * (kClassStride + 8) + 16 * sizeof(double);
@@ -467,3 +610,5 @@ extern "C" int piqmc_qa_colored(const piqmc_problem *pb, int R, int P, int S, in
         return run_colored<uint32_t>(pb, R, P, S, M, temp, jp, confs, seed, rbase, sbase, etrace, block, rpw, st);
     return run_colored<uint64_t>(pb, R, P, S, M, temp, jp, confs, seed, rbase, sbase, etrace, block, rpw, st);
 }
+
+#include "batch_part.inc"

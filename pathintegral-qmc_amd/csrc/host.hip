@@ -384,6 +384,13 @@ extern "C" int piqmc_problem_set_coloring(piqmc_problem *p, int nc, const int32_
     return upload_coloring(*p);
 }
 
+extern "C" int piqmc_problem_set_sweep_path(piqmc_problem *p, int path)
+{
+    if (!p || path < 0 || path > 2) return set_error(PIQMC_EINVAL, "sweep path must be 0 (auto), 1 or 2");
+    p->sweep_path = path;
+    return PIQMC_OK;
+}
+
 extern "C" int piqmc_problem_get_coloring(const piqmc_problem *p, int32_t *color)
 {
     if (!p || !color) return set_error(PIQMC_EINVAL, "bad arguments");

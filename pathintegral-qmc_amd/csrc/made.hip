@@ -245,15 +245,16 @@ __global__ void adam_b(float *b, float *m, float *v, const float *g, int n, floa
     b[t] -= lr * (mv / c1) / (sqrtf(vv / c2) + eps);
 }
 
-// Sampling step d: draw s_d for every row from sigmoid(logit d).
-__global__ void sample_step(const float *logits, int B, int N, int d, uint32_t k0, uint32_t k1, uint32_t round,
-                            int8_t *s, uint16_t *x)
+// Sampling step d: draw s_d for every row from sigmoid(logit d).  The key and
+// round come from device memory (rng = {key0, key1, round}) so that the
+// captured sampling graph stays valid across calls.
+__global__ void sample_step(const float *logits, int B, int N, int d, const uint32_t *rng, int8_t *s, uint16_t *x)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
     const float l = logits[(size_t)b * N + d];
     uint32_t out[4];
-    philox4x32_10(round, (uint32_t)b, (uint32_t)d, 0x5A4D0000u, k0, k1, out);
+    philox4x32_10(rng[2], (uint32_t)b, (uint32_t)d, 0x5A4D0000u, rng[0], rng[1], out);
     const double u = ((double)out[0] + 0.5) * 2.3283064365386963e-10;
     const double p = 1.0 / (1.0 + exp(-(double)l));  // p(s_d = -1)
     const bool neg = u < p;
@@ -279,6 +280,13 @@ struct piqmc_made {
     double *nll = nullptr;
     int8_t *s = nullptr;
     int step = 0;
+    // sampling graph (captured once per batch size on a private stream)
+    uint32_t *rng = nullptr;  // {key0, key1, round}
+    hipStream_t gs = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int gB = 0;
+    int use_graph = 1;
 };
 
 namespace {
@@ -291,6 +299,7 @@ int made_alloc_batch(piqmc_made *m, int B)
                      (void **)&m->s};
     for (auto p : bufs)
         if (*p) (void)hipFree(*p), *p = nullptr;
+    if (m->gexec) (void)hipGraphExecDestroy(m->gexec), m->gexec = nullptr, m->gB = 0;  // holds old addresses
     const int Hm = std::max(std::max(H1, H2), N);
     PIQMC_HIP(hipMalloc(&m->x, 2 * (size_t)B * N));
     PIQMC_HIP(hipMalloc(&m->h1, 2 * (size_t)B * H1));
@@ -393,9 +402,13 @@ extern "C" int piqmc_made_destroy(piqmc_made *m)
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
-    void *bs[] = {m->x, m->h1, m->h2, m->logits, m->dlog, m->dh2, m->dh1, m->tA, m->tB, m->nll, m->s};
+    void *bs[] = {m->x, m->h1, m->h2, m->logits, m->dlog, m->dh2, m->dh1, m->tA, m->tB, m->nll, m->s, m->rng};
     for (void *p : bs)
         if (p) (void)hipFree(p);
+    if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+    if (m->ev_in) (void)hipEventDestroy(m->ev_in);
+    if (m->ev_out) (void)hipEventDestroy(m->ev_out);
+    if (m->gs) (void)hipStreamDestroy(m->gs);
     delete m;
     return PIQMC_OK;
 }
@@ -509,15 +522,48 @@ extern "C" int piqmc_made_sample(piqmc_made *m, int B, uint64_t seed, uint32_t r
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     const int N = m->N;
-    fill_i8<<<grid1((size_t)B * N), 256, 0, st>>>(m->s, (size_t)B * N, 1);  // placeholder spins beyond the prefix
-    spins_to_bf16<<<grid1((size_t)B * N), 256, 0, st>>>(m->s, (size_t)B * N, m->x);
-    for (int d = 0; d < N; ++d) {
-        forward(m, B, st);
-        sample_step<<<(B + 255) / 256, 256, 0, st>>>(m->logits, B, N, d, (uint32_t)seed, (uint32_t)(seed >> 32),
-                                                     round, m->s, m->x);
+    if (!m->rng) PIQMC_HIP(hipMalloc(&m->rng, 4 * sizeof(uint32_t)));
+    const uint32_t rng_h[4] = {(uint32_t)seed, (uint32_t)(seed >> 32), round, 0};
+    PIQMC_HIP(hipMemcpyAsync(m->rng, rng_h, sizeof(rng_h), hipMemcpyHostToDevice, st));
+    PIQMC_HIP(hipStreamSynchronize(st));  // rng_h lives on this stack frame
+    // N dependent steps of (3 GEMMs + draw): the whole chain is one sequence
+    // of launches that only depends on B, so it is captured once into a HIP
+    // graph and replayed.
+    auto chain = [&](hipStream_t s) {
+        fill_i8<<<grid1((size_t)B * N), 256, 0, s>>>(m->s, (size_t)B * N, 1);  // placeholders beyond the prefix
+        spins_to_bf16<<<grid1((size_t)B * N), 256, 0, s>>>(m->s, (size_t)B * N, m->x);
+        for (int d = 0; d < N; ++d) {
+            forward(m, B, s);
+            sample_step<<<(B + 255) / 256, 256, 0, s>>>(m->logits, B, N, d, m->rng, m->s, m->x);
+        }
+        forward(m, B, s);
+        output_nll<<<B, 256, 0, s>>>(m->logits, m->s, B, N, 0.f, nullptr, m->nll, -1.0);
+    };
+    if (m->use_graph) {
+        if (!m->gs) {
+            PIQMC_HIP(hipStreamCreateWithFlags(&m->gs, hipStreamNonBlocking));
+            PIQMC_HIP(hipEventCreateWithFlags(&m->ev_in, hipEventDisableTiming));
+            PIQMC_HIP(hipEventCreateWithFlags(&m->ev_out, hipEventDisableTiming));
+        }
+        if (!m->gexec || m->gB != B) {
+            if (m->gexec) (void)hipGraphExecDestroy(m->gexec), m->gexec = nullptr;
+            hipGraph_t graph = nullptr;
+            PIQMC_HIP(hipStreamBeginCapture(m->gs, hipStreamCaptureModeThreadLocal));
+            chain(m->gs);
+            PIQMC_HIP(hipStreamEndCapture(m->gs, &graph));
+            hipError_t e = hipGraphInstantiate(&m->gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return piqmc::hip_fail(e, "sampling graph instantiate");
+            m->gB = B;
+        }
+        PIQMC_HIP(hipEventRecord(m->ev_in, st));
+        PIQMC_HIP(hipStreamWaitEvent(m->gs, m->ev_in, 0));
+        PIQMC_HIP(hipGraphLaunch(m->gexec, m->gs));
+        PIQMC_HIP(hipEventRecord(m->ev_out, m->gs));
+        PIQMC_HIP(hipStreamWaitEvent(st, m->ev_out, 0));
+    } else {
+        chain(st);
     }
-    forward(m, B, st);
-    output_nll<<<B, 256, 0, st>>>(m->logits, m->s, B, N, 0.f, nullptr, m->nll, -1.0);
     PIQMC_HIP(hipMemcpyAsync(s_out_dev, m->s, (size_t)B * N, hipMemcpyDeviceToDevice, st));
     if (logq_dev) PIQMC_HIP(hipMemcpyAsync(logq_dev, m->nll, 8 * (size_t)B, hipMemcpyDeviceToDevice, st));
     PIQMC_HIP(hipGetLastError());
@@ -607,6 +653,13 @@ extern "C" int piqmc_made_mh(piqmc_made *m, const piqmc_problem *pb, int R, int 
     PIQMC_HIP(hipFreeAsync(old, st));
     PIQMC_HIP(hipFreeAsync(lq_old, st));
     PIQMC_HIP(hipGetLastError());
+    return PIQMC_OK;
+}
+
+extern "C" int piqmc_made_set_graph(piqmc_made *m, int use_graph)
+{
+    if (!m) return piqmc::set_error(PIQMC_EINVAL, "null made");
+    m->use_graph = use_graph ? 1 : 0;
     return PIQMC_OK;
 }
 

@@ -41,6 +41,7 @@ struct piqmc_problem {
     int pmj = 0;
     int zmax = 0;                 // max nonzero off-diagonal entries per row
     int zuni = -1;                // common degree when every row has zmax entries, else -1
+    int sweep_path = 0;           // parallel sweep: 0 auto, 1 word per thread, 2 dense (lane per slice)
     int32_t *d_cnb = nullptr;     // [zmax][N] neighbour of position q (color order)
     uint32_t *d_cinfo = nullptr;  // [N] bits 0..7 negative-coupling mask, 8..11 z
 };

@@ -30,6 +30,7 @@ _SIGS = {
     "piqmc_problem_info": ([_vp, _vp, _vp, _vp, _vp, _vp], _i),
     "piqmc_problem_set_coloring": ([_vp, _i, _vp], _i),
     "piqmc_problem_get_coloring": ([_vp, _vp], _i),
+    "piqmc_problem_set_sweep_path": ([_vp, _i], _i),
     "piqmc_rand_seed": ([_u32, _vp], _i),
     "piqmc_rand_advance": ([_vp, _u64], _i),
     "piqmc_perm_streams": ([_vp, _i, _i, _i, _vp, _i], _i),
@@ -50,8 +51,14 @@ _SIGS = {
     "piqmc_made_logprob": ([_vp, _i, _vp, _vp, _vp], _i),
     "piqmc_made_train_step": ([_vp, _i, _vp, _f, _vp, _vp], _i),
     "piqmc_made_sample": ([_vp, _i, _u64, _u32, _vp, _vp, _vp], _i),
+    "piqmc_made_set_graph": ([_vp, _i], _i),
     "piqmc_made_mh": ([_vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _f, _f, _u64, _u32, _vp, _vp], _i),
     "piqmc_gemm_bf16_nt": ([_vp, _vp, _i, _i, _i, _vp, _vp], _i),
+    "piqmc_batch_create": ([C.POINTER(_vp)], _i),
+    "piqmc_batch_destroy": ([_vp], _i),
+    "piqmc_batch_add": ([_vp, _vp, _i, _i, _i, _i, _f, _vp, _vp, _u64, _u32, _u32, _vp], _i),
+    "piqmc_batch_run": ([_vp, _i, _i, _vp], _i),
+    "piqmc_batch_info": ([_vp, _vp, _vp, _vp], _i),
 }
 for _name, (_args, _res) in _SIGS.items():
     _fn = getattr(lib, _name)

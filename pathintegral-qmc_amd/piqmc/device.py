@@ -98,6 +98,12 @@ class Problem:
         _lib.call("piqmc_problem_get_coloring", self._h, out.ctypes.data)
         return out
 
+    def set_sweep_path(self, path):
+        """Parallel-sweep kernel: 'auto', 'word' (a parity word per thread) or
+        'dense' (a replica per wavefront, a lane per slice)."""
+        code = {"auto": 0, "word": 1, "dense": 2}[path]
+        _lib.call("piqmc_problem_set_sweep_path", self._h, code)
+
     def set_coloring(self, color):
         color = np.ascontiguousarray(color, np.int32)
         nc = int(color.max()) + 1
@@ -167,3 +173,48 @@ class Problem:
         _lib.call("piqmc_slice_energies", self._h, flat.shape[0], flat.data_ptr(), E.data_ptr(),
                   _stream(stream))
         return E.reshape(spins.shape[:-1])
+
+
+class Batch:
+    """Heterogeneous parallel sweeps in one persistent launch.
+
+    ``add`` takes the arguments of :meth:`Problem.qa_colored`; ``run`` sweeps
+    every job with one launch whose workgroups take work items from a device
+    queue.  Each job's spins (and energy trace) equal its own ``qa_colored``
+    call with the same seed, replica base and sweep base.
+    """
+
+    def __init__(self):
+        h = C.c_void_p()
+        _lib.call("piqmc_batch_create", C.byref(h))
+        self._h = h
+        self._keep = []
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                _lib.lib.piqmc_batch_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def add(self, problem, confs, sched, mcsteps, temp, seed, replica_base=0, sweep_base=0,
+            trace=False):
+        import torch
+        R, P, N = confs.shape
+        sched = np.ascontiguousarray(sched, np.float64)
+        _check_dev(confs, torch.int8, (R, P, problem.nspins), "confs")
+        et = torch.zeros((R, len(sched)), dtype=torch.float64, device=confs.device) if trace else None
+        _lib.call("piqmc_batch_add", self._h, problem._h, R, P, len(sched), int(mcsteps), float(temp),
+                  sched.ctypes.data, confs.data_ptr(), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                  int(replica_base), int(sweep_base), et.data_ptr() if trace else None)
+        self._keep.append((problem, confs, et))
+        return et
+
+    def run(self, block=0, rpw=0, stream=None):
+        _lib.call("piqmc_batch_run", self._h, int(block), int(rpw), _stream(stream))
+
+    def info(self):
+        v = [C.c_int() for _ in range(3)]
+        _lib.call("piqmc_batch_info", self._h, *[C.byref(x) for x in v])
+        return {"jobs": v[0].value, "items": v[1].value, "grid": v[2].value}

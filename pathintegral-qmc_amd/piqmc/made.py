@@ -85,6 +85,11 @@ class MADE:
             hist.append(tot / max(cnt, 1))
         return hist
 
+    def set_graph(self, use_graph):
+        """Run the sampler's N-step launch chain as a captured HIP graph
+        (default) or as eager launches."""
+        _lib.call("piqmc_made_set_graph", self._h, int(bool(use_graph)))
+
     def sample(self, B, seed, round_, device="cuda", stream=None):
         import torch
         s = torch.empty((B, self.nspins), dtype=torch.int8, device=device)

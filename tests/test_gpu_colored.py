@@ -13,11 +13,13 @@ from conftest import pm_j_problem
 pytestmark = pytest.mark.gpu
 
 
-def _run(gpu, nbs, P, sched, M, T, R, seed, rbase=0, sbase=0, block=0, rpw=0, coloring=None):
+def _run(gpu, nbs, P, sched, M, T, R, seed, rbase=0, sbase=0, block=0, rpw=0, coloring=None,
+         path="auto"):
     import torch
     from piqmc import device
     N = nbs.shape[0]
     pb = device.Problem(nbs)
+    pb.set_sweep_path(path)
     if coloring is not None:
         pb.set_coloring(coloring)
     color = pb.coloring()
@@ -146,3 +148,67 @@ def test_colored_statistics_vs_sequential_oracle(gpu):
     pg, ps = np.mean(Eg == e_best), np.mean(Es == e_best)
     pp = (pg + ps) / 2
     assert abs(pg - ps) <= 3 * np.sqrt(max(pp * (1 - pp), 1e-12) * 2 / nseed) + 1e-12, (pg, ps)
+
+
+@pytest.mark.parametrize("N,P", [(64, 16), (100, 32), (256, 128), (40, 20)])
+def test_dense_path_sk_bit_exact(gpu, N, P):
+    # Sherrington-Kirkpatrick couplings J_ij ~ N(0, 1/N): one color per spin,
+    # the lane-per-slice path, against the CPU twin
+    from piqmc import instances
+    n, r, c, v = instances.sk(N, np.random.RandomState(N))
+    nbs = instances.neighbors(n, r, c, v)
+    pb, got, ref, etg, etr = _run(gpu, nbs, P, np.linspace(3.0, 1e-8, 6), 1, 0.05, R=5, seed=N + P,
+                                  rbase=3)
+    assert pb.ncolors == N
+    assert np.array_equal(got, ref)
+    np.testing.assert_allclose(etg, etr, rtol=1e-12, atol=1e-10)
+
+
+@pytest.mark.parametrize("field_sigma", [0.0, 0.3])
+def test_dense_and_word_paths_agree(gpu, field_sigma):
+    N, nbs, _, _, _ = pm_j_problem(6, seed=12, field_sigma=field_sigma)
+    outs = []
+    for path in ("word", "dense"):
+        _, got, ref, _, _ = _run(gpu, nbs, 16, np.linspace(3.0, 1e-8, 10), 1, 0.05, R=6, seed=5,
+                                 path=path)
+        assert np.array_equal(got, ref), path
+        outs.append(got)
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_batch_heterogeneous_equals_individual_runs(gpu):
+    # one persistent launch over a mix of +-1 lattices (2-D and 3-D, 32- and
+    # 64-bit words), a field-carrying lattice and SK glasses, with different
+    # slice counts, schedule lengths and replica counts
+    import torch
+    from piqmc import device, instances
+    jobs = []
+    rng = np.random.RandomState(0)
+    specs = [("pm", 8, 2, 16, 40, 5), ("pm", 16, 2, 128, 12, 3), ("pm", 4, 3, 32, 25, 4),
+             ("field", 5, 2, 20, 15, 3), ("sk", 64, 0, 16, 10, 4), ("sk", 48, 0, 64, 8, 5),
+             ("pm", 6, 2, 2, 30, 7)]
+    for t, (kind, L, dim, P, S, R) in enumerate(specs):
+        if kind == "sk":
+            n, r, c, v = instances.sk(L, np.random.RandomState(t))
+        else:
+            n, r, c, v = instances.lattice_pm_j(L, dim, np.random.RandomState(t),
+                                                 0.2 if kind == "field" else 0.0)
+        pb = device.Problem(instances.neighbors(n, r, c, v))
+        c0 = (2 * rng.randint(2, size=(R, P, n)) - 1).astype(np.int8)
+        sched = np.linspace(3.0, 1e-8, S)
+        jobs.append((pb, c0, sched, 0.05 + 0.01 * t, 100 + t, 10 * t, t))
+    b = device.Batch()
+    outs = []
+    for pb, c0, sched, T, seed, rb, sb in jobs:
+        c8 = torch.tensor(c0, device=gpu)
+        et = b.add(pb, c8, sched, 1, T, seed, replica_base=rb, sweep_base=sb, trace=True)
+        outs.append((c8, et))
+    b.run()
+    torch.cuda.synchronize()
+    assert b.info()["jobs"] == len(jobs)
+    for (pb, c0, sched, T, seed, rb, sb), (c8, et) in zip(jobs, outs):
+        ref = torch.tensor(c0, device=gpu)
+        eref = pb.qa_colored(ref, sched, 1, T, seed, replica_base=rb, sweep_base=sb, trace=True)
+        torch.cuda.synchronize()
+        assert torch.equal(c8, ref)
+        torch.testing.assert_close(et, eref, rtol=0, atol=1e-12)

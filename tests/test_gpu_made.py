@@ -87,6 +87,29 @@ def test_sampler_density_is_the_evaluated_density(gpu):
     assert chi2 < dof + 5 * np.sqrt(2 * dof), (chi2, dof)
 
 
+def test_sampling_graph_replays_match_eager(gpu):
+    # the captured sampling graph, replayed several times with eager work and
+    # synchronisations in between, reproduces the eager launches every time
+    import torch
+    N = 16
+    m = _made(N, 24, 24, seed=7)
+    rng = np.random.RandomState(2)
+    data = torch.tensor((2 * rng.randint(2, size=(128, N)) - 1).astype(np.int8), device=gpu)
+    for _ in range(5):
+        m.train_step(data, 1e-2)
+    for rnd in range(4):
+        m.set_graph(False)
+        se, lqe = m.sample(300, seed=123, round_=rnd)
+        torch.cuda.synchronize()
+        m.train_step(data, 0.0)                 # unrelated eager work between replays
+        m.set_graph(True)
+        sg, lqg = m.sample(300, seed=123, round_=rnd)
+        torch.cuda.synchronize()
+        assert torch.equal(se, sg) and torch.equal(lqe, lqg), rnd
+    s2, _ = m.sample(300, seed=123, round_=1)
+    assert not torch.equal(s2, se)              # a different round draws differently
+
+
 def test_mh_move_leaves_boltzmann_law_invariant(gpu):
     # frustrated 3-spin, 4-slice system at fixed field: alternate parallel
     # sweeps and MH slice moves; the mean slice energy matches exact enumeration
