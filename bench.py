@@ -56,6 +56,48 @@ def cpu_baseline(nbs, N, P, T, sched, reps):
         reps, len(sched), att)
 
 
+def batch_leg(n_inst=512, reps=4, seed=0):
+    """512 instances mixing 2-D +-J lattices L in {8,16,32,64} and SK glasses
+    N in {64,128,256} (J ~ N(0,1/N)), P in {16,32,64,128}, schedule lengths
+    uniform in [100,1000], T = 1/P: one persistent launch."""
+    import numpy as np
+    import torch
+    from piqmc import device, instances
+    rng = np.random.RandomState(seed)
+    kinds = [("pm", 8), ("pm", 16), ("pm", 32), ("pm", 64), ("sk", 64), ("sk", 128), ("sk", 256)]
+    probs, b, attempts, mix = {}, device.Batch(), 0, {}
+    for t in range(n_inst):
+        kind, size = kinds[rng.randint(len(kinds))]
+        P = int(rng.choice([16, 32, 64, 128]))
+        S = int(rng.randint(100, 1001))
+        key = (kind, size, t % 4)          # a few distinct instances per size
+        if key not in probs:
+            iseed = (1000 if kind == "sk" else 0) + 10 * size + t % 4
+            if kind == "sk":
+                n, r, c, v = instances.sk(size, np.random.RandomState(iseed))
+            else:
+                n, r, c, v = instances.lattice_pm_j(size, 2, np.random.RandomState(iseed))
+            probs[key] = device.Problem(instances.neighbors(n, r, c, v))
+        pb = probs[key]
+        g = torch.Generator(device="cpu").manual_seed(t)
+        confs = (torch.randint(0, 2, (reps, P, pb.nspins), generator=g, dtype=torch.int8) * 2 - 1).cuda()
+        b.add(pb, confs, np.linspace(3.0, 1e-8, S), 1, 1.0 / P, 1000 + t, replica_base=t * reps)
+        attempts += reps * P * pb.nspins * S
+        mix[kind] = mix.get(kind, 0) + 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    from piqmc import device as D
+    info = b.info()
+    return {"attempts_per_s": attempts / dt, "seconds": dt, "kernel_ms": D.last_kernel_ms(),
+            "instances": n_inst, "replicas_per_instance": reps, "mix": mix,
+            "work_items": info["items"], "persistent_workgroups": info["grid"],
+            "workload": "512 instances: 2-D +-J L in {8,16,32,64} and SK N in {64,128,256}, "
+                        "P in {16,32,64,128}, S ~ U[100,1000], one persistent launch"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +111,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-replay", action="store_true")
     ap.add_argument("--replay-steps", type=int, default=25)
+    ap.add_argument("--no-batch", action="store_true", help="skip the heterogeneous batch leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the sweep shards replicas with no exchange, so "
@@ -175,6 +219,15 @@ def main():
                          "kernel_ms": rk * 1e3, "field_steps": Sr, "replicas": R,
                          "near_ties": int(ties.sum())}
         del perms
+
+    if not args.no_batch:
+        out["batch"] = batch_leg()
+    if not args.no_learned:
+        from piqmc import learned
+        lr = learned.run(eval_steps=(25, 50, 100, 200))
+        out["learned"] = {k: lr[k] for k in ("workload", "train_slices", "train_nll_per_epoch",
+                                             "train_seconds", "e_best_per_spin", "curves",
+                                             "time_ratio")}
 
     if not args.no_cpu:
         single, omp, nthr, sample = cpu_baseline(nbs, N, P, np.float32(T), sched, reps=2)

@@ -111,7 +111,9 @@ __device__ __forceinline__ WT ge_pick(int k, WT c0, WT c1, WT c2)
 }
 
 // Replicas r0 .. r0+nrep-1 of one problem, whole schedule, by one workgroup.
-template <typename WT, bool PMJ>
+// UZ: every spin has the same degree a.zuni, so the class thresholds of a
+// field step are wavefront-uniform and the mask selection is scalar code.
+template <typename WT, bool PMJ, bool UZ = false>
 __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned char *smem)
 {
     const int N = a.N, W = a.W, tid = threadIdx.x, bd = blockDim.x;
@@ -131,7 +133,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
         }
         __syncthreads();
         // uniform-degree lattices: the class thresholds of this step are wave-uniform
-        const uint32_t zt_u = (PMJ && a.zuni >= 0) ? a.ztab[s * 8 + a.zuni] : 0u;
+        const uint32_t zt_u = (PMJ && UZ) ? __builtin_amdgcn_readfirstlane(a.ztab[s * 8 + a.zuni]) : 0u;
         const float jp = a.jperp[s];
         for (int m = 0; m < a.M; ++m) {
             const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
@@ -145,12 +147,12 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                         int nb[8];
                         if (PMJ) {
                             info = a.cinfo[q];
-                            zi = (info >> 8) & 15;
+                            zi = UZ ? a.zuni : (int)((info >> 8) & 15);
 #pragma unroll
                             for (int z = 0; z < 8; ++z)
                                 if (z < a.zmax) nb[z] = z < zi ? a.cnb[(size_t)z * N + q] : i;
                         }
-                        const uint32_t zt = (PMJ && a.zuni >= 0) ? zt_u : (PMJ ? tz[zi] : 0u);
+                        const uint32_t zt = UZ ? zt_u : (PMJ ? tz[zi] : 0u);
                         for (int rr = 0; rr < nrep; ++rr) {
                             WT *mine = st + ((size_t)rr * 2 + tau) * N;
                             const WT *other = st + ((size_t)rr * 2 + 1 - tau) * N;
@@ -315,12 +317,12 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
     for (int q = tid; q < 2 * N * nrep; q += bd) gst[q] = st[q];
 }
 
-template <typename WT, bool PMJ>
+template <typename WT, bool PMJ, bool UZ>
 __global__ void colored_kernel(ColArgs<WT> a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int r0 = blockIdx.x * a.rpw;
-    colored_body<WT, PMJ>(a, r0, min(a.rpw, a.R - r0), smem);
+    colored_body<WT, PMJ, UZ>(a, r0, min(a.rpw, a.R - r0), smem);
 }
 
 // ------------------------------------------------------------------ dense path
@@ -571,11 +573,16 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
     const int grid = (R + rpw - 1) / rpw;
     piqmc::timing_begin(st);
     if (pmj) {
-        (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        colored_kernel<WT, true><<<grid, block, lds, st>>>(a);
+        if (a.zuni >= 0) {
+            (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            colored_kernel<WT, true, true><<<grid, block, lds, st>>>(a);
+        } else {
+            (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            colored_kernel<WT, true, false><<<grid, block, lds, st>>>(a);
+        }
     } else {
-        (void)hipFuncSetAttribute((const void *)colored_kernel<WT, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        colored_kernel<WT, false><<<grid, block, lds, st>>>(a);
+        (void)hipFuncSetAttribute((const void *)colored_kernel<WT, false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        colored_kernel<WT, false, false><<<grid, block, lds, st>>>(a);
     }
     piqmc::timing_end(st);
     PIQMC_HIP(hipGetLastError());
