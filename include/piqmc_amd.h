@@ -51,6 +51,14 @@ typedef struct piqmc_problem piqmc_problem;
 
 int piqmc_problem_create(int nspins, int maxnb, const double *nbs_host,
                          piqmc_problem **out);
+/* A problem from a dense coupling matrix J (element (a,b) at J[a*s0 + b*s1],
+ * float64; diagonal = fields; only the upper triangle is read), the argument
+ * of qmc.QuantumAnneal_dense / sa.Anneal_dense (piqmc/qmc.pyx:148,
+ * piqmc/sa.pyx:129).  Replays on such a problem follow the *_dense functions:
+ * couplings in double precision, and sa replay accepts ediff > 0
+ * (piqmc/sa.pyx:181) where sa.Anneal accepts ediff >= 0. */
+int piqmc_problem_create_dense(int nspins, const double *J_host, int64_t s0, int64_t s1,
+                               piqmc_problem **out);
 int piqmc_problem_destroy(piqmc_problem *pb);
 /* ncolors: spatial colors found; pm_j: 1 if every coupling is +-1 between
  * distinct spins (bit-sliced path), zmax: max couplings per spin. */
@@ -114,6 +122,18 @@ int piqmc_quantum_anneal(const double *sched, int S, int mcsteps, int slices, fl
 int piqmc_anneal(const double *sched, int S, int mcsteps, double *svec, int64_t ss, int nspins,
                  const double *nbs, int maxnb, const int64_t *perms, uint32_t *rand_state32,
                  uint64_t *draws, uint32_t *ties);
+/* qmc.QuantumAnneal_dense (piqmc/qmc.pyx:141-242): as piqmc_quantum_anneal
+ * with the dense matrix J (element (a,b) at J[a*js0 + b*js1]) in place of
+ * the neighbour table. */
+int piqmc_quantum_anneal_dense(const double *sched, int S, int mcsteps, int slices, float temp,
+                               int nspins, double *confs, int64_t cs0, int64_t cs1,
+                               const double *J, int64_t js0, int64_t js1, const int64_t *perms,
+                               uint32_t *rand_state32, uint64_t *draws, uint32_t *ties);
+/* sa.Anneal_dense (piqmc/sa.pyx:126-187). */
+int piqmc_anneal_dense(const double *sched, int S, int mcsteps, double *svec, int64_t ss,
+                       int nspins, const double *J, int64_t js0, int64_t js1,
+                       const int64_t *perms, uint32_t *rand_state32, uint64_t *draws,
+                       uint32_t *ties);
 
 /* ------------------------------------------------------------------ */
 /* Parallel sweep ("fixed" semantics: periodic Trotter neighbours,     */
@@ -129,6 +149,15 @@ int piqmc_anneal(const double *sched, int S, int mcsteps, double *svec, int64_t 
 int piqmc_qa_colored(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
                      const double *sched_host, int8_t *confs_dev, uint64_t seed,
                      uint32_t replica_base, uint32_t sweep_base, double *etrace_dev,
+                     int block, int rpw, void *stream);
+/* Classical annealing with the parallel sweep (the preanneal of
+ * examples/spinglass32.py:121, sa.Anneal, on the GPU): C independent chains
+ * spins_dev int8 [C][N] (C even) in place, temperature sched_host[s] in field
+ * step s, M sweeps per step.  Fixed semantics and Philox draws as
+ * piqmc_qa_colored: chain c is slice c % P' of replica replica_base + c / P'
+ * (P' the largest even divisor of C up to 64) with zero Trotter coupling. */
+int piqmc_sa_colored(const piqmc_problem *pb, int C, int S, int M, const double *sched_host,
+                     int8_t *spins_dev, uint64_t seed, uint32_t replica_base, uint32_t sweep_base,
                      int block, int rpw, void *stream);
 
 /* ------------------------------------------------------------------ */

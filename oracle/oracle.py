@@ -45,6 +45,12 @@ def lib():
                                     i64p, u32p, C.POINTER(C.c_uint64)]
         L.orc_sa_compat.argtypes = [f64p, C.c_int, C.c_int, C.c_void_p, C.c_long, C.c_int,
                                     f64p, C.c_int, i64p, u32p, C.POINTER(C.c_uint64)]
+        L.orc_qa_dense.argtypes = [f64p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
+                                   C.c_void_p, C.c_long, C.c_long, C.c_void_p, C.c_long, C.c_long,
+                                   i64p, u32p, C.POINTER(C.c_uint64)]
+        L.orc_sa_dense.argtypes = [f64p, C.c_int, C.c_int, C.c_void_p, C.c_long, C.c_int,
+                                   C.c_void_p, C.c_long, C.c_long, i64p, u32p,
+                                   C.POINTER(C.c_uint64)]
         L.orc_qa_parallel.argtypes = [f64p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
                                       C.c_void_p, C.c_long, C.c_long, f64p, C.c_int, C.c_int]
         L.orc_qa_fixed_seq.argtypes = [f64p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
@@ -53,11 +59,14 @@ def lib():
         L.orc_qa_colored.argtypes = [f64p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
                                      i8p, f64p, C.c_int, i32p, C.c_int, C.c_uint64,
                                      C.c_uint32, C.c_uint32, C.c_void_p]
+        L.orc_sa_colored.argtypes = [f64p, C.c_int, C.c_int, C.c_int, C.c_int, i8p, f64p,
+                                     C.c_int, i32p, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32]
         L.orc_energy_nbs.argtypes = [i8p, C.c_int, C.c_int, f64p, C.c_int, f64p]
         L.orc_energy_coo.argtypes = [f64p, C.c_int, C.c_long, i64p, i64p, f64p]
         L.orc_energy_coo.restype = C.c_double
-        for name in ("orc_qa_compat", "orc_sa_compat", "orc_qa_parallel",
-                     "orc_qa_fixed_seq", "orc_qa_colored"):
+        for name in ("orc_qa_compat", "orc_sa_compat", "orc_qa_dense", "orc_sa_dense",
+                     "orc_qa_parallel",
+                     "orc_qa_fixed_seq", "orc_qa_colored", "orc_sa_colored"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -143,6 +152,37 @@ def sa_compat(sched, mcsteps, svec, nbs, perms, rstate):
     return d.value
 
 
+def qa_dense(sched, mcsteps, P, temp, N, confs, J, perms, rstate):
+    """qmc.QuantumAnneal_dense restatement; confs f64 [N,P] and J f64 [N,N]
+    (any strides).  Returns the rand() draw count."""
+    sched = np.ascontiguousarray(sched, np.float64)
+    perms = np.ascontiguousarray(perms, np.int64)
+    assert confs.dtype == np.float64 and J.dtype == np.float64
+    cs0, cs1 = _strides(confs)
+    js0, js1 = _strides(J)
+    d = C.c_uint64(0)
+    rc = lib().orc_qa_dense(sched, len(sched), int(mcsteps), int(P), float(temp), int(N),
+                            confs.ctypes.data, cs0, cs1, J.ctypes.data, js0, js1, perms,
+                            rstate.st, C.byref(d))
+    if rc:
+        raise ValueError("orc_qa_dense rc=%d" % rc)
+    return d.value
+
+
+def sa_dense(sched, mcsteps, svec, J, perms, rstate):
+    sched = np.ascontiguousarray(sched, np.float64)
+    perms = np.ascontiguousarray(perms, np.int64)
+    assert svec.dtype == np.float64 and J.dtype == np.float64
+    js0, js1 = _strides(J)
+    d = C.c_uint64(0)
+    rc = lib().orc_sa_dense(sched, len(sched), int(mcsteps), svec.ctypes.data,
+                            svec.strides[0] // 8, svec.shape[0], J.ctypes.data, js0, js1,
+                            perms, rstate.st, C.byref(d))
+    if rc:
+        raise ValueError("orc_sa_dense rc=%d" % rc)
+    return d.value
+
+
 def qa_parallel(sched, mcsteps, P, temp, N, confs, nbs, nthreads):
     sched = np.ascontiguousarray(sched, np.float64)
     nbs = np.ascontiguousarray(nbs, np.float64)
@@ -183,6 +223,20 @@ def qa_colored(sched, mcsteps, P, temp, spins, nbs, color, ncolors, seed, replic
     if rc:
         raise ValueError("orc_qa_colored rc=%d" % rc)
     return et
+
+
+def sa_colored(sched, mcsteps, spins, nbs, color, ncolors, seed, replica_base=0, sweep_base=0):
+    """Parallel-sweep classical annealing of the chains spins int8 [C, N]
+    (C even) in place; see orc_sa_colored for the chain layout."""
+    sched = np.ascontiguousarray(sched, np.float64)
+    nbs = np.ascontiguousarray(nbs, np.float64)
+    color = np.ascontiguousarray(color, np.int32)
+    assert spins.dtype == np.int8 and spins.flags.c_contiguous and spins.ndim == 2
+    rc = lib().orc_sa_colored(sched, len(sched), int(mcsteps), spins.shape[0], spins.shape[1],
+                              spins, nbs, nbs.shape[1], color, int(ncolors),
+                              int(seed) & 0xFFFFFFFFFFFFFFFF, int(replica_base), int(sweep_base))
+    if rc:
+        raise ValueError("orc_sa_colored rc=%d" % rc)
 
 
 def energy_nbs(spins, nbs):

@@ -19,6 +19,10 @@
  *   orc_qa_compat     : hand traces on N=2..4 (tests/golden), defect D1/D2
  *                       reductions, Boixo outcome test (testing/test_boixo.py:90-114).
  *   orc_sa_compat     : hand traces, Boixo SA outcome test (test_boixo.py:72-88).
+ *   orc_qa_dense,     : equal to orc_qa_compat / orc_sa_compat (and so to their
+ *   orc_sa_dense        pins) on matrices whose entries are float-exact, given
+ *                       the neighbour table of the same matrix; a hand trace
+ *                       where double-precision couplings round differently.
  *   orc_qa_fixed_seq  : exact enumeration of the stationary law on tiny
  *                       systems (N=2,P=2) at fixed Gamma.
  *   orc_qa_colored    : the same stationarity check; brute-force equality
@@ -274,6 +278,102 @@ int orc_sa_compat(const double *sched, int S, int M, double *svec, long ss, int 
 }
 
 /* ------------------------------------------------------------------ */
+/* qmc.QuantumAnneal_dense, piqmc/qmc.pyx:197-242 (qmc.c:2957-3390):   */
+/* every si in 0..N-1 in order, J read as double from the upper        */
+/* triangle (J[sidx,si] if sidx<si else J[si,sidx], diagonal = field), */
+/* each term added in double and stored to the float ediff; D1 and D2  */
+/* as QuantumAnneal.  J element (a,b) at J[a*js0 + b*js1].             */
+/* ------------------------------------------------------------------ */
+int orc_qa_dense(const double *sched, int S, int M, int P, float temp, int N,
+                 double *confs, long cs0, long cs1, const double *J, long js0, long js1,
+                 const int64_t *perms, uint32_t *rs, uint64_t *draws)
+{
+    if (P < 2 || N < 1) return -1;
+    uint64_t nd = 0;
+    for (int s = 0; s < S; ++s) {
+        float jperp = orc_jperp(sched[s], P, temp);
+        for (int m = 0; m < M; ++m) {
+            const int64_t *perm = perms + ((size_t)s * M + m) * N;
+            for (int k = 0; k < P; ++k) {
+                float ediff = 0.0f;
+                const int tleft = P - 1, tright = 1;   /* D1: tidx stays 0 */
+                for (int t = 0; t < N; ++t) {
+                    int sidx = (int)perm[t];
+                    double ci = confs[sidx * cs0 + k * cs1];
+                    for (int si = 0; si < N; ++si) {
+                        if (si == sidx)
+                            ediff = (float)((double)ediff + (-2.0 * ci) * J[sidx * js0 + si * js1]);
+                        else if (sidx < si)
+                            ediff = (float)((double)ediff + (-2.0 * ci) *
+                                            (J[sidx * js0 + si * js1] * confs[si * cs0 + k * cs1]));
+                        else
+                            ediff = (float)((double)ediff + (-2.0 * ci) *
+                                            (J[si * js0 + sidx * js1] * confs[si * cs0 + k * cs1]));
+                    }
+                    ediff = (float)((double)ediff + (-2.0 * ci) *
+                                    ((double)jperp * confs[sidx * cs0 + tleft * cs1]));
+                    ediff = (float)((double)ediff + (-2.0 * ci) *
+                                    ((double)jperp * confs[sidx * cs0 + tright * cs1]));
+                    if ((double)ediff > 0.0) {
+                        confs[sidx * cs0 + k * cs1] *= -1.0;
+                    } else {
+                        uint32_t r = orc_rand_next(rs);
+                        ++nd;
+                        if (exp((double)(ediff / temp)) > (double)r / (double)ORC_RAND_MAX)
+                            confs[sidx * cs0 + k * cs1] *= -1.0;
+                    }
+                }
+            }
+        }
+    }
+    if (draws) *draws = nd;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* sa.Anneal_dense, piqmc/sa.pyx:160-187 (sa.c:3287-3490): per-spin    */
+/* reset, accept ediff > 0 (:181), temp = (float)sched[t] (:163).      */
+/* ------------------------------------------------------------------ */
+int orc_sa_dense(const double *sched, int S, int M, double *svec, long ss, int N,
+                 const double *J, long js0, long js1, const int64_t *perms, uint32_t *rs,
+                 uint64_t *draws)
+{
+    if (N < 1) return -1;
+    uint64_t nd = 0;
+    for (int s = 0; s < S; ++s) {
+        float temp = (float)sched[s];
+        for (int m = 0; m < M; ++m) {
+            const int64_t *perm = perms + ((size_t)s * M + m) * N;
+            for (int t = 0; t < N; ++t) {
+                int sidx = (int)perm[t];
+                float ediff = 0.0f;
+                double ci = svec[sidx * ss];
+                for (int si = 0; si < N; ++si) {
+                    if (si == sidx)
+                        ediff = (float)((double)ediff + (-2.0 * ci) * J[sidx * js0 + si * js1]);
+                    else if (sidx < si)
+                        ediff = (float)((double)ediff + (-2.0 * ci) *
+                                        (J[sidx * js0 + si * js1] * svec[si * ss]));
+                    else
+                        ediff = (float)((double)ediff + (-2.0 * ci) *
+                                        (J[si * js0 + sidx * js1] * svec[si * ss]));
+                }
+                if ((double)ediff > 0.0) {
+                    svec[sidx * ss] *= -1.0;
+                } else {
+                    uint32_t r = orc_rand_next(rs);
+                    ++nd;
+                    if (exp((double)(ediff / temp)) > (double)r / (double)ORC_RAND_MAX)
+                        svec[sidx * ss] *= -1.0;
+                }
+            }
+        }
+    }
+    if (draws) *draws = nd;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* qmc.QuantumAnneal_parallel, piqmc/qmc.pyx:247-357: OpenMP guided    */
 /* prange over spins, float64 ediffs[] (reset after each slice, :357), */
 /* no permutation, D1 kept, libc rand() shared by all threads.  Timing */
@@ -432,10 +532,10 @@ static void orc_sliced_words(uint32_t g, uint32_t i, uint32_t rep, uint32_t tau,
         }
     }
 }
-int orc_qa_colored(const double *sched, int S, int M, int P, float temp, int N,
-                   int8_t *sp, const double *nbs, int Z, const int32_t *color,
-                   int ncolors, uint64_t seed, uint32_t replica, uint32_t sweep_base,
-                   double *etrace)
+static int colored_core(const float *jps, const float *temps, int S, int M, int P, int N,
+                        int8_t *sp, const double *nbs, int Z, const int32_t *color,
+                        int ncolors, uint64_t seed, uint32_t replica, uint32_t sweep_base,
+                        double *etrace)
 {
     if (P < 2 || (P & 1) || N < 1 || Z < 1 || ncolors < 1) return -1;
     for (int i = 0; i < N; ++i) {
@@ -449,7 +549,7 @@ int orc_qa_colored(const double *sched, int S, int M, int P, float temp, int N,
     }
     const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     for (int s = 0; s < S; ++s) {
-        float jperp = orc_jperp(sched[s], P, temp);
+        const float jperp = jps[s], temp = temps[s];
         for (int m = 0; m < M; ++m) {
             uint32_t g = sweep_base + (uint32_t)(s * M + m);
             for (int tau = 0; tau < 2; ++tau) {
@@ -484,6 +584,48 @@ int orc_qa_colored(const double *sched, int S, int M, int P, float temp, int N,
         if (etrace) etrace[s] = orc_mean_energy(sp, P, N, nbs, Z);
     }
     return 0;
+}
+
+int orc_qa_colored(const double *sched, int S, int M, int P, float temp, int N,
+                   int8_t *sp, const double *nbs, int Z, const int32_t *color,
+                   int ncolors, uint64_t seed, uint32_t replica, uint32_t sweep_base,
+                   double *etrace)
+{
+    if (P < 2) return -1;
+    float *jps = (float *)malloc(sizeof(float) * (S > 0 ? 2 * S : 1));
+    if (!jps) return -4;
+    for (int s = 0; s < S; ++s) {
+        jps[s] = orc_jperp(sched[s], P, temp);
+        jps[S + s] = temp;
+    }
+    int rc = colored_core(jps, jps + S, S, M, P, N, sp, nbs, Z, color, ncolors, seed, replica,
+                          sweep_base, etrace);
+    free(jps);
+    return rc;
+}
+
+/* Classical annealing of C chains with the parallel sweep: chain c is slice
+ * c % P of replica rbase + c / P (P the largest even divisor of C up to 64),
+ * with zero Trotter coupling and temperature (float)sched[s] in step s. */
+int orc_sa_colored(const double *sched, int S, int M, int C, int N, int8_t *spins,
+                   const double *nbs, int Z, const int32_t *color, int ncolors,
+                   uint64_t seed, uint32_t rbase, uint32_t sweep_base)
+{
+    if (C < 2 || (C & 1)) return -1;
+    int P = 64;
+    while (C % P) P -= 2;
+    float *jps = (float *)malloc(sizeof(float) * (S > 0 ? 2 * S : 1));
+    if (!jps) return -4;
+    for (int s = 0; s < S; ++s) {
+        jps[s] = 0.0f;
+        jps[S + s] = (float)sched[s];
+    }
+    int rc = 0;
+    for (int q = 0; q < C / P && !rc; ++q)
+        rc = colored_core(jps, jps + S, S, M, P, N, spins + (size_t)q * P * N, nbs, Z, color,
+                          ncolors, seed, rbase + (uint32_t)q, sweep_base, NULL);
+    free(jps);
+    return rc;
 }
 
 /* sa.ClassicalIsingEnergy, piqmc/sa.pyx:41-44:  -s.(J_off s) - sum(diag(J) s)

@@ -32,9 +32,9 @@ constexpr int kClassStride = 8 * 8 * 3;  // (z, u_s, u_t) thresholds per field s
 template <typename WT>
 struct ColArgs {
     int N, Z, P, W, S, M, ncolors, zmax, R, rpw, zuni;
-    float temp;
     uint32_t key0, key1, replica_base, sweep_base;
     const float *jperp;        // [S]
+    const float *temps;        // [S] temperature of each field step
     const int2 *rows;          // [N][Z]
     const int32_t *order;      // [N] spins by color
     const int *color_off;      // [ncolors+1]
@@ -135,6 +135,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
         // uniform-degree lattices: the class thresholds of this step are wave-uniform
         const uint32_t zt_u = (PMJ && UZ) ? __builtin_amdgcn_readfirstlane(a.ztab[s * 8 + a.zuni]) : 0u;
         const float jp = a.jperp[s];
+        const double Ts = (double)a.temps[s];
         for (int m = 0; m < a.M; ++m) {
             const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
             for (int tau = 0; tau < 2; ++tau) {
@@ -255,7 +256,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                     const double ts = (((Lw >> mm) & 1) ? -1.0 : 1.0) + (((Rw >> mm) & 1) ? -1.0 : 1.0);
                                     const double ed = -2.0 * si * (h + (double)jp * ts);
                                     unsigned int thr = 0;
-                                    const int cls = piqmc_fixed_class(ed, (double)a.temp, &thr);
+                                    const int cls = piqmc_fixed_class(ed, Ts, &thr);
                                     bool fl = cls == 1;
                                     if (cls == 2) {
                                         if (!have_r) {  // the word's 32 bit-sliced random words
@@ -345,6 +346,7 @@ __device__ void dense_replica(const ColArgs<WT> &a, int r, WT *st)
     for (int q = lane; q < 2 * N; q += 64) st[q] = gst[q];
     for (int s = 0; s < a.S; ++s) {
         const float jp = a.jperp[s];
+        const double Ts = (double)a.temps[s];
         for (int m = 0; m < a.M; ++m) {
             const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
             for (int tau = 0; tau < 2; ++tau) {
@@ -369,7 +371,7 @@ __device__ void dense_replica(const ColArgs<WT> &a, int r, WT *st)
                     const double ts = (((O >> ml) & one) ? -1.0 : 1.0) + (((O >> mr) & one) ? -1.0 : 1.0);
                     const double ed = -2.0 * si * (h + (double)jp * ts);
                     unsigned int thr = 0;
-                    const int cls = piqmc_fixed_class(ed, (double)a.temp, &thr);
+                    const int cls = piqmc_fixed_class(ed, Ts, &thr);
                     bool fl = active && cls == 1;
                     if (__ballot(active && cls == 2)) {
                         constexpr int kPer = sizeof(WT) == 4 ? 4 : 2;
@@ -466,8 +468,8 @@ int grid_for(size_t n, int block)
 // nibbles (amin, smin) at bits 0,4 / 8,12 / 16,20; thr holds the draw
 // thresholds.  Returns false if some class is not monotone (then the per-bit
 // path is used).
-bool pmj_tables(const piqmc_problem *pb, int S, const std::vector<float> &jp, float temp,
-                std::vector<uint32_t> &thr, std::vector<uint32_t> &ztab)
+bool pmj_tables(const piqmc_problem *pb, int S, const std::vector<float> &jp,
+                const std::vector<float> &temps, std::vector<uint32_t> &thr, std::vector<uint32_t> &ztab)
 {
     thr.assign((size_t)S * kClassStride, 0);
     ztab.assign((size_t)S * 8, 0);
@@ -479,7 +481,7 @@ bool pmj_tables(const piqmc_problem *pb, int S, const std::vector<float> &jp, fl
                     // -2 s_i (h + Jp*ts) with s_i*h = z - 2u_s, s_i*ts = 2 - 2u_t (exact)
                     const double ed = -2.0 * ((double)(z - 2 * us) + (double)jp[s] * (double)(2 - 2 * ut));
                     unsigned int t = 0;
-                    cls[us] = piqmc_fixed_class(ed, (double)temp, &t);
+                    cls[us] = piqmc_fixed_class(ed, (double)temps[s], &t);
                     if (cls[us] == 2) thr[(size_t)s * kClassStride + (z * 8 + us) * 3 + ut] = t;
                 }
                 int amin = z + 1, smin = z + 1;
@@ -499,8 +501,8 @@ bool use_dense(const piqmc_problem *pb)
 }
 
 template <typename WT>
-int run_colored(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
-                const std::vector<float> &jp, int8_t *confs, uint64_t seed, uint32_t rbase,
+int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
+                const std::vector<float> &jp, const std::vector<float> &temps, int8_t *confs, uint64_t seed, uint32_t rbase,
                 uint32_t sbase, double *etrace, int block, int rpw, hipStream_t st)
 {
     const int N = pb->N;
@@ -511,15 +513,16 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
         const int wpb = block / 64;
         const size_t lds = sizeof(WT) * 2 * (size_t)N * wpb;
         if (lds > 160 * 1024) return piqmc::set_error(PIQMC_EINVAL, "dense sweep: state exceeds LDS");
-        float *d_jp = nullptr;
+        float *d_jp = nullptr;  // [S] J_perp, then [S] temperatures
         WT *d_packed = nullptr;
-        PIQMC_HIP(hipMallocAsync((void **)&d_jp, sizeof(float) * (S ? S : 1), st));
+        PIQMC_HIP(hipMallocAsync((void **)&d_jp, sizeof(float) * (S ? 2 * S : 1), st));
         if (S) PIQMC_HIP(hipMemcpyAsync(d_jp, jp.data(), sizeof(float) * S, hipMemcpyHostToDevice, st));
+        if (S) PIQMC_HIP(hipMemcpyAsync(d_jp + S, temps.data(), sizeof(float) * S, hipMemcpyHostToDevice, st));
         PIQMC_HIP(hipMallocAsync((void **)&d_packed, sizeof(WT) * 2 * (size_t)N * R, st));
         pack_parity_kernel<WT><<<grid_for((size_t)R * 2 * N, 256), 256, 0, st>>>(confs, d_packed, R, P, N);
         ColArgs<WT> a = {};
         a.N = N; a.Z = pb->Z; a.P = P; a.W = P / 2; a.S = S; a.M = M; a.R = R;
-        a.ncolors = pb->ncolors; a.temp = temp;
+        a.ncolors = pb->ncolors; a.temps = d_jp + S;
         a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
         a.replica_base = rbase; a.sweep_base = sbase;
         a.jperp = d_jp; a.rows = pb->d_rows; a.order = pb->d_order;
@@ -536,7 +539,7 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
         return PIQMC_OK;
     }
     std::vector<uint32_t> thr, ztab;
-    const bool pmj = pb->pmj && pmj_tables(pb, S, jp, temp, thr, ztab);
+    const bool pmj = pb->pmj && pmj_tables(pb, S, jp, temps, thr, ztab);
     if (rpw <= 0) rpw = R >= 2048 ? 4 : (R >= 1024 ? 2 : 1);
     if (rpw > 64) return piqmc::set_error(PIQMC_EINVAL, "replicas per workgroup must be <= 64");
     const size_t lds = sizeof(WT) * 2 * (size_t)N * rpw + 4 * (kClassStride + 8) + 16 * sizeof(double);
@@ -544,12 +547,13 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
         return piqmc::set_error(PIQMC_EINVAL, "colored sweep: replicas per workgroup x 2 x nspins x P/2 bits exceed one workgroup's LDS");
     if (block <= 0) block = 256;
     if (block > 1024 || (block & 63)) return piqmc::set_error(PIQMC_EINVAL, "block must be a multiple of 64, <= 1024");
-    float *d_jp = nullptr;
+    float *d_jp = nullptr;  // [S] J_perp, then [S] temperatures
     uint32_t *d_thr = nullptr, *d_zt = nullptr;
     int *d_coff = nullptr;
     WT *d_packed = nullptr;
-    PIQMC_HIP(hipMallocAsync((void **)&d_jp, sizeof(float) * (S ? S : 1), st));
+    PIQMC_HIP(hipMallocAsync((void **)&d_jp, sizeof(float) * (S ? 2 * S : 1), st));
     if (S) PIQMC_HIP(hipMemcpyAsync(d_jp, jp.data(), sizeof(float) * S, hipMemcpyHostToDevice, st));
+    if (S) PIQMC_HIP(hipMemcpyAsync(d_jp + S, temps.data(), sizeof(float) * S, hipMemcpyHostToDevice, st));
     if (pmj && S) {
         PIQMC_HIP(hipMallocAsync((void **)&d_thr, sizeof(uint32_t) * thr.size(), st));
         PIQMC_HIP(hipMallocAsync((void **)&d_zt, sizeof(uint32_t) * ztab.size(), st));
@@ -563,7 +567,7 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
 
     ColArgs<WT> a;
     a.N = N; a.Z = pb->Z; a.P = P; a.W = P / 2; a.S = S; a.M = M;
-    a.ncolors = pb->ncolors; a.zmax = pb->zmax; a.temp = temp;
+    a.ncolors = pb->ncolors; a.zmax = pb->zmax; a.temps = d_jp + S;
     a.R = R; a.rpw = rpw; a.zuni = pmj ? pb->zuni : -1;
     a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
     a.replica_base = rbase; a.sweep_base = sbase;
@@ -613,9 +617,33 @@ extern "C" int piqmc_qa_colored(const piqmc_problem *pb, int R, int P, int S, in
     }
     if ((long)S * M > 0 && temp == 0.0f) return piqmc::set_error(PIQMC_EZERODIV, "float division");
     hipStream_t st = (hipStream_t)stream;
+    const std::vector<float> temps(jp.size(), temp);
     if (P / 2 <= 32)
-        return run_colored<uint32_t>(pb, R, P, S, M, temp, jp, confs, seed, rbase, sbase, etrace, block, rpw, st);
-    return run_colored<uint64_t>(pb, R, P, S, M, temp, jp, confs, seed, rbase, sbase, etrace, block, rpw, st);
+        return run_colored<uint32_t>(pb, R, P, S, M, jp, temps, confs, seed, rbase, sbase, etrace, block, rpw, st);
+    return run_colored<uint64_t>(pb, R, P, S, M, jp, temps, confs, seed, rbase, sbase, etrace, block, rpw, st);
+}
+
+// Classical annealing of C independent chains (C even) with the parallel
+// sweep: chains are laid out as the slices of C/P' replicas (P' the largest
+// even divisor of C up to 64) whose Trotter coupling is exactly zero, so each
+// chain sees only its spatial couplings, at temperature sched[s] in step s.
+extern "C" int piqmc_sa_colored(const piqmc_problem *pb, int C, int S, int M, const double *sched,
+                                int8_t *spins, uint64_t seed, uint32_t rbase, uint32_t sbase,
+                                int block, int rpw, void *stream)
+{
+    if (!pb || C < 2 || (C & 1) || S < 0 || M < 0 || (S > 0 && !sched) || !spins)
+        return piqmc::set_error(PIQMC_EINVAL, "sa_colored: bad arguments (the chain count must be even)");
+    int P = 64;
+    while (C % P) P -= 2;
+    std::vector<float> jp(S > 0 ? S : 0, 0.0f), temps(S > 0 ? S : 0);
+    for (int s = 0; s < S; ++s) {
+        temps[s] = (float)sched[s];
+        if (!(temps[s] > 0.0f) || !std::isfinite(temps[s]))
+            return piqmc::set_error(PIQMC_EINVAL, "sa_colored: temperatures must be finite and > 0");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    return run_colored<uint32_t>(pb, C / P, P, S, M, jp, temps, spins, seed, rbase, sbase, nullptr, block,
+                                 rpw, st);
 }
 
 #include "batch_part.inc"

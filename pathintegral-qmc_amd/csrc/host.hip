@@ -5,7 +5,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <thread>
+#include <utility>
 #include <unordered_map>
 
 #include "piqmc_internal.h"
@@ -345,10 +347,57 @@ extern "C" int piqmc_problem_create(int N, int Z, const double *nbs, piqmc_probl
     return PIQMC_OK;
 }
 
+// Rows of a dense coupling matrix as the *_dense functions read it
+// (piqmc/qmc.pyx:206-219, piqmc/sa.pyx:169-179): spin i meets every si in
+// ascending order, J[i,i] as its field, J[min(i,si), max(i,si)] otherwise (the
+// lower triangle is never read).  A zero coupling adds +-0 to the running
+// float ediff, which changes neither its value nor any later comparison, so
+// zero entries are dropped and the surviving terms keep their order.
+extern "C" int piqmc_problem_create_dense(int N, const double *Jm, int64_t s0, int64_t s1,
+                                          piqmc_problem **out)
+{
+    if (!out) return set_error(PIQMC_EINVAL, "null output handle");
+    *out = nullptr;
+    if (N < 1 || !Jm) return set_error(PIQMC_EINVAL, "nspins must be >= 1 and J non-null");
+    std::vector<std::vector<std::pair<int, double>>> rows(N);
+    int Z = 1;
+    for (int i = 0; i < N; ++i) {
+        for (int si = 0; si < N; ++si) {
+            const int a = std::min(i, si), b = std::max(i, si);
+            const double v = Jm[a * s0 + b * s1];
+            if (!std::isfinite(v) || !std::isfinite((float)v))
+                return set_error(PIQMC_EINVAL, "non-finite coupling J[" + std::to_string(a) + "," +
+                                                   std::to_string(b) + "]");
+            if (v != 0.0) rows[i].push_back({si, v});
+        }
+        Z = std::max(Z, (int)rows[i].size());
+    }
+    std::vector<double> nbs((size_t)N * Z * 2, 0.0), jd((size_t)N * Z, 0.0);
+    for (int i = 0; i < N; ++i)
+        for (size_t e = 0; e < rows[i].size(); ++e) {
+            nbs[((size_t)i * Z + e) * 2] = rows[i][e].first;
+            nbs[((size_t)i * Z + e) * 2 + 1] = rows[i][e].second;
+            jd[(size_t)i * Z + e] = rows[i][e].second;
+        }
+    piqmc_problem *p = nullptr;
+    int rc = piqmc_problem_create(N, Z, nbs.data(), &p);
+    if (rc) return rc;
+    p->dense = 1;
+    hipError_t e = hipMalloc(&p->d_jd, sizeof(double) * jd.size());
+    if (e == hipSuccess) e = hipMemcpy(p->d_jd, jd.data(), sizeof(double) * jd.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        piqmc_problem_destroy(p);
+        return piqmc::hip_fail(e, "problem upload");
+    }
+    *out = p;
+    return PIQMC_OK;
+}
+
 extern "C" int piqmc_problem_destroy(piqmc_problem *p)
 {
     if (!p) return PIQMC_OK;
     if (p->d_rows) (void)hipFree(p->d_rows);
+    if (p->d_jd) (void)hipFree(p->d_jd);
     if (p->d_order) (void)hipFree(p->d_order);
     if (p->d_cnb) (void)hipFree(p->d_cnb);
     if (p->d_cinfo) (void)hipFree(p->d_cinfo);

@@ -30,6 +30,10 @@ struct piqmc_problem {
     std::vector<int32_t> idx;  // [N][Z]  (int)nbs[i][z][0]
     std::vector<float> J;      // [N][Z]  (float)nbs[i][z][1]
     int2 *d_rows = nullptr;    // [N][Z]  (j, float bits of J), replay + generic path
+    // Problems built from a dense coupling matrix replay the *_dense functions,
+    // which read J in double precision: d_jd holds those values, [N][Z].
+    int dense = 0;
+    double *d_jd = nullptr;
 
     // Graph coloring of the nonzero off-diagonal couplings.
     int ncolors = 0;

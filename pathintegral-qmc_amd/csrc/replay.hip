@@ -4,13 +4,15 @@
 // A replica's sweep is a strictly sequential Markov chain (the reference keeps
 // a running ediff across the spins of a slice, D2, and draws libc rand() only
 // on the rejection branch), so one lane owns one replica and the replicas of
-// a workgroup advance in lockstep.  Per lane the kernel keeps, bit-packed in
-// LDS with lane-interleaved words (conflict-free):
+// a wave advance in lockstep.  A few lanes per wave own replicas (a chain's
+// speed is its latency, so replicas are spread over every SIMD) and the waves
+// of a workgroup share one LDS copy of the neighbour rows.  Per replica the
+// kernel keeps, bit-packed in LDS with slot-interleaved words (conflict-free):
 //   - the slice being swept (streamed from HBM once per slice),
 //   - slices 1 and P-1, the only Trotter neighbours the reference reads (D1),
 //   - the 31-word glibc TYPE_3 ring of its rand() stream.
-// The permutation stream is read from HBM as [sweep][position][replica] uint16,
-// one coalesced 2*L-byte load per attempt.
+// The permutation stream is read from HBM as [sweep][position][replica]
+// uint16, eight attempts ahead of use so its latency leaves the chain.
 //
 // Arithmetic is the reference's: ediff is float and every term is an exact
 // +-2*float product, so the float add equals the reference's double add
@@ -18,15 +20,18 @@
 // comparison is against rand()/2147483647.0 in double.
 #include <math.h>
 
+#include <algorithm>
+
 #include "piqmc_internal.h"
 
 namespace {
 
 struct ReplayArgs {
-    int R, N, P, Z, S, M, Wn, sa;
+    int R, N, P, Z, S, M, Wn, sa, rows_lds, Lw;
     float temp;
     const float *steps;     // [S] J_perp (QA) or float temperature (SA)
     const int2 *rows;       // [N][Z] (j, J bits)
+    const double *jd;       // [N][Z] double J (problems built from a dense matrix), else null
     const uint16_t *perms;  // [S*M][N][R]
     uint32_t *bits;         // [P][Wn][R]
     uint32_t *rstate;       // [R][32]
@@ -39,13 +44,29 @@ __device__ __forceinline__ uint32_t getbit(const uint32_t *reg, int L, int lane,
     return (reg[(i >> 5) * L + lane] >> (i & 31)) & 1u;
 }
 
+template <bool F64J>
 __global__ void replay_kernel(ReplayArgs a)
 {
     extern __shared__ uint32_t sm[];
-    const int L = blockDim.x, lane = threadIdx.x;
+    // The first Lw lanes of each wave own one replica each ("slots"); a
+    // replica's sweep is a dependency chain, so few lanes per wave spread the
+    // replicas over every SIMD instead of stacking them in a few waves.
+    const int Lw = a.Lw, L = (blockDim.x >> 6) * Lw;  // L: slots per workgroup (LDS word stride)
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    const int lane = wv * Lw + ln;
     const int r = blockIdx.x * L + lane;
-    if (r >= a.R) return;  // no barriers below: each lane touches only its own words
     const int N = a.N, P = a.P, Z = a.Z, Wn = a.Wn, R = a.R;
+    const int nreg = a.sa ? 1 : (P == 2 ? 2 : 3);
+    // neighbour rows staged in LDS (every thread helps, then the only barrier)
+    const int2 *rows = a.rows;
+    if (a.rows_lds) {
+        // rows start on an 8-byte boundary after the spin words and the rand() rings
+        int2 *lr = reinterpret_cast<int2 *>(sm + (((size_t)(nreg * Wn + 31) * L + 1) & ~(size_t)1));
+        for (int q = threadIdx.x; q < N * Z; q += blockDim.x) lr[q] = a.rows[q];
+        __syncthreads();
+        rows = lr;
+    }
+    if (ln >= Lw || r >= a.R) return;  // no barriers below: a slot touches only its own words
     uint32_t *cur = sm, *s1, *sp, *ring;
     if (a.sa) {
         s1 = sp = cur;
@@ -53,7 +74,7 @@ __global__ void replay_kernel(ReplayArgs a)
     } else {
         s1 = sm + Wn * L;
         sp = (P == 2) ? s1 : sm + 2 * Wn * L;
-        ring = sm + 3 * Wn * L;
+        ring = sm + nreg * Wn * L;  // nreg spin regions, then the rand() rings
     }
     auto load = [&](uint32_t *reg, int k) {
         for (int w = 0; w < Wn; ++w) reg[w * L + lane] = a.bits[((size_t)k * Wn + w) * R + r];
@@ -73,6 +94,7 @@ __global__ void replay_kernel(ReplayArgs a)
     uint32_t nt = 0;
     const float T = a.temp;
     const int nslice = a.sa ? 1 : P;
+    constexpr int kAhead = 8;  // permutation entries loaded one block ahead
 
     for (int s = 0; s < a.S; ++s) {
         const float sv = a.steps[s];
@@ -84,46 +106,75 @@ __global__ void replay_kernel(ReplayArgs a)
                 const bool own = !a.sa && reg == cur;
                 if (own) load(cur, k);
                 float e = 0.0f;
-                int inext = pg[0];
-                for (int t = 0; t < N; ++t) {
-                    const int i = inext;
-                    if (t + 1 < N) inext = pg[(size_t)(t + 1) * R];
-                    const float m2 = getbit(reg, L, lane, i) ? 2.0f : -2.0f;  // -2*s_i
-                    if (a.sa) e = 0.0f;
-                    const int2 *row = a.rows + (size_t)i * Z;
-                    for (int z = 0; z < Z; ++z) {
-                        const int2 rw = row[z];
-                        const float J = __int_as_float(rw.y);
-                        if (rw.x == i) {
-                            e = e + m2 * J;
+                int blk[kAhead];
+#pragma unroll
+                for (int j = 0; j < kAhead; ++j) blk[j] = j < N ? pg[(size_t)j * R] : 0;
+                for (int t0 = 0; t0 < N; t0 += kAhead) {
+                    int nxt[kAhead];
+#pragma unroll
+                    for (int j = 0; j < kAhead; ++j) {
+                        const int t = t0 + kAhead + j;
+                        nxt[j] = t < N ? pg[(size_t)t * R] : 0;
+                    }
+#pragma unroll
+                    for (int j = 0; j < kAhead; ++j) {
+                        if (t0 + j >= N) break;
+                        const int i = blk[j];
+                        const float m2 = getbit(reg, L, lane, i) ? 2.0f : -2.0f;  // -2*s_i
+                        if (a.sa) e = 0.0f;
+                        const int2 *row = rows + (size_t)i * Z;
+                        if constexpr (F64J) {
+                            // *_dense: J is read as double, so each term is
+                            // added in double and the sum rounded to float
+                            // (piqmc/qmc.pyx:209-218, piqmc/sa.pyx:172-179)
+                            const double *jrow = a.jd + (size_t)i * Z;
+                            for (int z = 0; z < Z; ++z) {
+                                const int j = row[z].x;
+                                const double J = jrow[z];
+                                const double t = j == i ? (double)m2 * J
+                                                        : (double)m2 * (getbit(reg, L, lane, j) ? -J : J);
+                                e = (float)((double)e + t);
+                            }
                         } else {
-                            const float sj = getbit(reg, L, lane, rw.x) ? -1.0f : 1.0f;
-                            e = e + m2 * (J * sj);
+                            for (int z = 0; z < Z; ++z) {
+                                const int2 rw = row[z];
+                                const float J = __int_as_float(rw.y);
+                                if (rw.x == i) {
+                                    e = e + m2 * J;
+                                } else {
+                                    const float sj = getbit(reg, L, lane, rw.x) ? -1.0f : 1.0f;
+                                    e = e + m2 * (J * sj);
+                                }
+                            }
                         }
+                        bool flip;
+                        if (!a.sa) {
+                            const float sl = getbit(sp, L, lane, i) ? -1.0f : 1.0f;  // slice P-1
+                            e = e + m2 * (sv * sl);
+                            const float sr = getbit(s1, L, lane, i) ? -1.0f : 1.0f;  // slice 1
+                            e = e + m2 * (sv * sr);
+                            flip = e > 0.0f;
+                        } else {
+                            // sa.Anneal accepts ediff >= 0 (piqmc/sa.pyx:114),
+                            // Anneal_dense ediff > 0 (piqmc/sa.pyx:181)
+                            flip = F64J ? e > 0.0f : e >= 0.0f;
+                        }
+                        if (!flip) {
+                            const uint32_t rp = f >= 3 ? f - 3 : f + 28;
+                            const uint32_t v = ring[f * L + lane] + ring[rp * L + lane];
+                            ring[f * L + lane] = v;
+                            f = f == 30 ? 0 : f + 1;
+                            ++nd;
+                            const double u = (double)(v >> 1) / 2147483647.0;
+                            const float q = e / tdiv;
+                            const double ex = exp((double)q);
+                            flip = ex > u;
+                            if (ex > 0.0 && ex < INFINITY && fabs(ex - u) <= ex * 0x1p-50) ++nt;
+                        }
+                        if (flip) reg[(i >> 5) * L + lane] ^= 1u << (i & 31);
                     }
-                    bool flip;
-                    if (!a.sa) {
-                        const float sl = getbit(sp, L, lane, i) ? -1.0f : 1.0f;  // slice P-1
-                        e = e + m2 * (sv * sl);
-                        const float sr = getbit(s1, L, lane, i) ? -1.0f : 1.0f;  // slice 1
-                        e = e + m2 * (sv * sr);
-                        flip = e > 0.0f;
-                    } else {
-                        flip = e >= 0.0f;
-                    }
-                    if (!flip) {
-                        const uint32_t rp = f >= 3 ? f - 3 : f + 28;
-                        const uint32_t v = ring[f * L + lane] + ring[rp * L + lane];
-                        ring[f * L + lane] = v;
-                        f = f == 30 ? 0 : f + 1;
-                        ++nd;
-                        const double u = (double)(v >> 1) / 2147483647.0;
-                        const float q = e / tdiv;
-                        const double ex = exp((double)q);
-                        flip = ex > u;
-                        if (ex > 0.0 && fabs(ex - u) <= ex * 0x1p-50) ++nt;
-                    }
-                    if (flip) reg[(i >> 5) * L + lane] ^= 1u << (i & 31);
+#pragma unroll
+                    for (int j = 0; j < kAhead; ++j) blk[j] = nxt[j];
                 }
                 if (own) store(cur, k);
             }
@@ -194,12 +245,28 @@ int run_replay(const piqmc_problem *pb, int R, int P, int S, int M, float temp, 
 {
     const int N = pb->N, Wn = (N + 31) / 32;
     const int nreg = sa ? 1 : (P == 2 ? 2 : 3);
-    // Lanes per workgroup: the widest power of two whose LDS image fits.
-    int L = 64;
-    auto lds = [&](int l) { return (size_t)(nreg * Wn + 31) * l * 4; };
-    while (L > 1 && (lds(L) > 160 * 1024 || L / 2 >= R)) L /= 2;
-    if (lds(L) > 160 * 1024)
+    // Launch shape: W waves per workgroup share one LDS copy of the neighbour
+    // rows; Lw lanes per wave each own a replica.  Lw is the smallest power of
+    // two that still gives every CU a workgroup, so replicas spread over all
+    // SIMDs; rows move to global memory only when they cannot fit.
+    const size_t rows_bytes = (size_t)N * pb->Z * sizeof(int2);
+    auto lds_of = [&](int slots, bool rows) {
+        const size_t words = (size_t)(nreg * Wn + 31) * slots;
+        return rows ? ((words + 1) & ~(size_t)1) * 4 + rows_bytes : words * 4;
+    };
+    constexpr size_t kLds = 160 * 1024;
+    constexpr int kCUs = 256;
+    int W = 4, Lw = 64;
+    while (Lw > 1 && (size_t)W * Lw * kCUs > (size_t)R) Lw /= 2;
+    bool rows_lds = lds_of(W * Lw, true) <= kLds;
+    while (lds_of(W * Lw, rows_lds) > kLds && (W > 1 || Lw > 1)) {
+        if (Lw > 1) Lw /= 2; else W /= 2;
+        if (!rows_lds && lds_of(W * Lw, true) <= kLds) rows_lds = true;
+    }
+    const int L = W * Lw;
+    if (lds_of(L, rows_lds) > kLds)
         return piqmc::set_error(PIQMC_EINVAL, "replay: nspins too large for one workgroup's LDS");
+    auto lds = [&](int l) { return lds_of(l, rows_lds); };
     const size_t nperm = (size_t)S * M * N * R;
     if (nperm && N < 65536) {
         int *d_bad = nullptr, bad = 0;
@@ -220,6 +287,8 @@ int run_replay(const piqmc_problem *pb, int R, int P, int S, int M, float temp, 
     pack_lanes_kernel<<<grid_for((size_t)R * P * Wn, 256), 256, 0, st>>>(confs, d_bits, R, P, N, Wn);
     ReplayArgs a;
     a.R = R; a.N = N; a.P = P; a.Z = pb->Z; a.S = S; a.M = M; a.Wn = Wn; a.sa = sa;
+    a.rows_lds = rows_lds ? 1 : 0;
+    a.Lw = Lw;
     a.temp = temp;
     a.steps = d_steps;
     a.rows = pb->d_rows;
@@ -228,10 +297,11 @@ int run_replay(const piqmc_problem *pb, int R, int P, int S, int M, float temp, 
     a.rstate = rand;
     a.draws = d_draws;
     a.ties = ties;
-    (void)hipFuncSetAttribute((const void *)replay_kernel,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds(L));
+    a.jd = pb->dense ? pb->d_jd : nullptr;
+    auto kern = pb->dense ? replay_kernel<true> : replay_kernel<false>;
+    (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds(L));
     piqmc::timing_begin(st);
-    replay_kernel<<<(R + L - 1) / L, L, lds(L), st>>>(a);
+    kern<<<(R + L - 1) / L, 64 * W, lds(L), st>>>(a);
     piqmc::timing_end(st);
     PIQMC_HIP(hipGetLastError());
     unpack_lanes_kernel<<<grid_for((size_t)R * P * N, 256), 256, 0, st>>>(d_bits, confs, R, P, N, Wn);
@@ -288,11 +358,20 @@ struct DevBuf {
     hipError_t alloc(size_t n) { return hipMalloc((void **)&p, sizeof(T) * (n ? n : 1)); }
 };
 
+// Couplings of a drop-in call: a neighbour table (QuantumAnneal, Anneal) or a
+// dense matrix with element strides (QuantumAnneal_dense, Anneal_dense).
+struct Couplings {
+    const double *nbs = nullptr;
+    int Z = 0;
+    const double *J = nullptr;
+    int64_t js0 = 0, js1 = 0;
+};
+
 int dropin(const double *sched, int S, int M, int P, float temp, int N, double *confs, int64_t cs0,
-           int64_t cs1, const double *nbs, int Z, const int64_t *perms, uint32_t *rs,
+           int64_t cs1, const Couplings &cp, const int64_t *perms, uint32_t *rs,
            uint64_t *draws_out, uint32_t *ties_out, int sa)
 {
-    if (N < 1 || Z < 1 || S < 0 || M < 0 || !confs || !nbs || !rs)
+    if (N < 1 || S < 0 || M < 0 || !confs || !rs || (cp.J ? false : (!cp.nbs || cp.Z < 1)))
         return piqmc::set_error(PIQMC_EINVAL, "bad arguments");
     if (!sa && P < 2) return piqmc::set_error(PIQMC_EINVAL, "QuantumAnneal needs slices >= 2 (reads slice 1)");
     if (rs[31] > 30) return piqmc::set_error(PIQMC_EINVAL, "bad TYPE_3 state");
@@ -313,7 +392,8 @@ int dropin(const double *sched, int S, int M, int P, float temp, int N, double *
             p16[g * N + t] = (uint16_t)v;
         }
     piqmc_problem *pb = nullptr;
-    int rc = piqmc_problem_create(N, Z, nbs, &pb);
+    int rc = cp.J ? piqmc_problem_create_dense(N, cp.J, cp.js0, cp.js1, &pb)
+                  : piqmc_problem_create(N, cp.Z, cp.nbs, &pb);
     if (rc) return rc;
     struct Guard {
         piqmc_problem *p;
@@ -354,14 +434,46 @@ extern "C" int piqmc_quantum_anneal(const double *sched, int S, int mcsteps, int
                                     const double *nbs, int maxnb, const int64_t *perms,
                                     uint32_t *rs, uint64_t *draws, uint32_t *ties)
 {
-    return dropin(sched, S, mcsteps, slices, temp, nspins, confs, cs0, cs1, nbs, maxnb, perms, rs,
-                  draws, ties, 0);
+    Couplings cp;
+    cp.nbs = nbs;
+    cp.Z = maxnb;
+    return dropin(sched, S, mcsteps, slices, temp, nspins, confs, cs0, cs1, cp, perms, rs, draws,
+                  ties, 0);
+}
+
+extern "C" int piqmc_quantum_anneal_dense(const double *sched, int S, int mcsteps, int slices,
+                                          float temp, int nspins, double *confs, int64_t cs0,
+                                          int64_t cs1, const double *J, int64_t js0, int64_t js1,
+                                          const int64_t *perms, uint32_t *rs, uint64_t *draws,
+                                          uint32_t *ties)
+{
+    if (!J) return piqmc::set_error(PIQMC_EINVAL, "null coupling matrix");
+    Couplings cp;
+    cp.J = J;
+    cp.js0 = js0;
+    cp.js1 = js1;
+    return dropin(sched, S, mcsteps, slices, temp, nspins, confs, cs0, cs1, cp, perms, rs, draws,
+                  ties, 0);
 }
 
 extern "C" int piqmc_anneal(const double *sched, int S, int mcsteps, double *svec, int64_t ss,
                             int nspins, const double *nbs, int maxnb, const int64_t *perms,
                             uint32_t *rs, uint64_t *draws, uint32_t *ties)
 {
-    return dropin(sched, S, mcsteps, 1, 0.0f, nspins, svec, ss, 0, nbs, maxnb, perms, rs, draws,
-                  ties, 1);
+    Couplings cp;
+    cp.nbs = nbs;
+    cp.Z = maxnb;
+    return dropin(sched, S, mcsteps, 1, 0.0f, nspins, svec, ss, 0, cp, perms, rs, draws, ties, 1);
+}
+
+extern "C" int piqmc_anneal_dense(const double *sched, int S, int mcsteps, double *svec, int64_t ss,
+                                  int nspins, const double *J, int64_t js0, int64_t js1,
+                                  const int64_t *perms, uint32_t *rs, uint64_t *draws, uint32_t *ties)
+{
+    if (!J) return piqmc::set_error(PIQMC_EINVAL, "null coupling matrix");
+    Couplings cp;
+    cp.J = J;
+    cp.js0 = js0;
+    cp.js1 = js1;
+    return dropin(sched, S, mcsteps, 1, 0.0f, nspins, svec, ss, 0, cp, perms, rs, draws, ties, 1);
 }

@@ -26,6 +26,7 @@ _SIGS = {
     "piqmc_version": ([], _i),
     "piqmc_last_kernel_ms": ([_vp], _i),
     "piqmc_problem_create": ([_i, _i, _vp, C.POINTER(_vp)], _i),
+    "piqmc_problem_create_dense": ([_i, _vp, _i64, _i64, C.POINTER(_vp)], _i),
     "piqmc_problem_destroy": ([_vp], _i),
     "piqmc_problem_info": ([_vp, _vp, _vp, _vp, _vp, _vp], _i),
     "piqmc_problem_set_coloring": ([_vp, _i, _vp], _i),
@@ -39,6 +40,10 @@ _SIGS = {
     "piqmc_quantum_anneal": ([_vp, _i, _i, _i, _f, _i, _vp, _i64, _i64, _vp, _i, _vp, _vp, _vp,
                               _vp], _i),
     "piqmc_anneal": ([_vp, _i, _i, _vp, _i64, _i, _vp, _i, _vp, _vp, _vp, _vp], _i),
+    "piqmc_quantum_anneal_dense": ([_vp, _i, _i, _i, _f, _i, _vp, _i64, _i64, _vp, _i64, _i64,
+                                    _vp, _vp, _vp, _vp], _i),
+    "piqmc_anneal_dense": ([_vp, _i, _i, _vp, _i64, _i, _vp, _i64, _i64, _vp, _vp, _vp, _vp], _i),
+    "piqmc_sa_colored": ([_vp, _i, _i, _i, _vp, _vp, _u64, _u32, _u32, _i, _i, _vp], _i),
     "piqmc_qa_colored": ([_vp, _i, _i, _i, _i, _f, _vp, _vp, _u64, _u32, _u32, _vp, _i, _i, _vp],
                          _i),
     "piqmc_slice_energies": ([_vp, _i, _vp, _vp, _vp], _i),

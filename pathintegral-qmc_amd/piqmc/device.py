@@ -8,6 +8,8 @@ already live on the GPU (torch is used only for device memory and streams):
                      (piqmc/qmc.pyx:30-136);
 * ``sa_replay``   -- sa.Anneal for R replicas (piqmc/sa.pyx:50-120);
 * ``qa_colored``  -- the graph-colored, parity-alternated parallel sweep;
+* ``sa_colored``  -- the same sweep as classical annealing (zero Trotter
+                     coupling, a temperature per step);
 * ``slice_energies`` -- classical energy of every slice.
 
 Spin tensors are int8 +1/-1: replicas x slices x spins ([R, P, N]).
@@ -77,8 +79,30 @@ class Problem:
         h = C.c_void_p()
         _lib.call("piqmc_problem_create", self.nspins, self.maxnb, nbs.ctypes.data, C.byref(h))
         self._h = h
+        self._load_info()
+
+    @classmethod
+    def from_dense(cls, J):
+        """A problem from a dense coupling matrix (float64 [N, N]; diagonal =
+        fields; the upper triangle is read), the argument of
+        QuantumAnneal_dense / Anneal_dense.  Replays on it follow those
+        functions (double-precision couplings; sa replay accepts ediff > 0)."""
+        J = np.asarray(J)
+        if J.dtype != np.float64 or J.ndim != 2 or J.shape[0] != J.shape[1]:
+            raise ValueError("J must be a square float64 matrix")
+        self = cls.__new__(cls)
+        self.nspins = J.shape[0]
+        h = C.c_void_p()
+        s0, s1 = (s // 8 for s in J.strides)
+        _lib.call("piqmc_problem_create_dense", self.nspins, J.ctypes.data, s0, s1, C.byref(h))
+        self._h = h
+        self._load_info()
+        return self
+
+    def _load_info(self):
         vals = [C.c_int() for _ in range(5)]
         _lib.call("piqmc_problem_info", self._h, *[C.byref(v) for v in vals])
+        self.maxnb = vals[1].value
         self.ncolors, self.pm_j, self.zmax = vals[2].value, bool(vals[3].value), vals[4].value
 
     def close(self):
@@ -162,6 +186,18 @@ class Problem:
                   int(replica_base), int(sweep_base), et.data_ptr() if trace else None,
                   int(block), int(rpw), _stream(stream))
         return et
+
+    def sa_colored(self, spins, sched, mcsteps, seed, replica_base=0, sweep_base=0, block=0,
+                   rpw=0, stream=None):
+        """Parallel-sweep classical annealing of the chains spins int8 [C, N]
+        (C even) in place, temperature sched[s] in step s."""
+        import torch
+        C, N = spins.shape
+        sched = np.ascontiguousarray(sched, np.float64)
+        _check_dev(spins, torch.int8, (C, self.nspins), "spins")
+        _lib.call("piqmc_sa_colored", self._h, C, len(sched), int(mcsteps), sched.ctypes.data,
+                  spins.data_ptr(), int(seed) & 0xFFFFFFFFFFFFFFFF, int(replica_base),
+                  int(sweep_base), int(block), int(rpw), _stream(stream))
 
     # ------------------------------------------------------------ energies
     def slice_energies(self, spins, stream=None):

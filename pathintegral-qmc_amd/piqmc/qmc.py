@@ -6,6 +6,9 @@ the same permutations are drawn from the caller's ``rng`` (1 + S*mcsteps
 ``rng.permutation`` calls, piqmc/qmc.pyx:90-91,136) and the same libc
 ``rand()`` stream is consumed and advanced (piqmc/qmc.pyx:132).
 
+``QuantumAnneal_dense`` (piqmc/qmc.pyx:141-242) is the same drop-in for a
+dense coupling matrix.
+
 ``QuantumAnneal_parallel`` keeps the signature of the reference's OpenMP
 variant (piqmc/qmc.pyx:247-357) and runs the race-free graph-colored,
 parity-alternated parallel sweep ("fixed" semantics: periodic Trotter
@@ -32,10 +35,26 @@ def QuantumAnneal(sched, mcsteps, slices, temp, nspins, confs, nbs, rng):
     slices) in place on ``confs`` [nspins, slices] (+1/-1 float64, any
     strides), piqmc/qmc.pyx:30-136.  Returns None.
     """
+    return _quantum_anneal(sched, mcsteps, slices, temp, nspins, confs, nbs, rng, dense=False)
+
+
+def QuantumAnneal_dense(sched, mcsteps, slices, temp, nspins, confs, J, rng):
+    """QuantumAnneal_dense(sched, mcsteps, slices, temp, nspins, confs, J, rng)
+
+    QuantumAnneal with a dense coupling matrix ``J`` (float64, off-diagonal
+    couplings, diagonal fields; only the upper triangle is read) in place of
+    the neighbour table, piqmc/qmc.pyx:141-242: the couplings enter the
+    energy change in double precision and every spin's row is scanned in
+    ascending order.  Returns None.
+    """
+    return _quantum_anneal(sched, mcsteps, slices, temp, nspins, confs, J, rng, dense=True)
+
+
+def _quantum_anneal(sched, mcsteps, slices, temp, nspins, confs, cpl, rng, dense):
     global last_near_ties, last_draws
     sched = f64_view(sched, 1, "sched")
     confs = f64_view(confs, 2, "confs", writable=True)
-    nbs = f64_view(nbs, 3, "nbs")
+    cpl = f64_view(cpl, 2 if dense else 3, "J" if dense else "nbs")
     mcsteps, slices, nspins = c_int(mcsteps), c_int(slices), c_int(nspins)
     temp = c_float(temp)
     S = sched.shape[0]
@@ -55,7 +74,11 @@ def QuantumAnneal(sched, mcsteps, slices, temp, nspins, confs, nbs, rng):
     if confs.shape[0] != nspins or confs.shape[1] != slices:
         raise ValueError("confs must have shape (nspins, slices) = (%d, %d), got %s"
                          % (nspins, slices, confs.shape))
-    if nbs.shape[0] != nspins:
+    if dense:
+        if cpl.shape[0] < nspins or cpl.shape[1] < nspins:
+            raise ValueError("J must be at least (nspins, nspins) = (%d, %d), got %s"
+                             % (nspins, nspins, cpl.shape))
+    elif cpl.shape[0] != nspins:
         raise ValueError("nbs must have nspins rows")
     if slices < 2:
         raise ValueError("QuantumAnneal needs slices >= 2 (the reference reads slice 1)")
@@ -65,14 +88,20 @@ def QuantumAnneal(sched, mcsteps, slices, temp, nspins, confs, nbs, rng):
     for g in range(nsweeps):
         perms[g] = perm
         perm = rng.permutation(perm)
-    nbs_c = np.ascontiguousarray(nbs)
     sched_c = np.ascontiguousarray(sched)
     st = _crand.get_state()
     draws, ties = C.c_uint64(0), C.c_uint32(0)
     cs0, cs1 = elem_strides(confs)
-    _lib.call("piqmc_quantum_anneal", sched_c.ctypes.data, S, mcsteps, slices, temp, nspins,
-              confs.ctypes.data, cs0, cs1, nbs_c.ctypes.data, nbs_c.shape[1],
-              perms.ctypes.data, st.ctypes.data, C.byref(draws), C.byref(ties))
+    if dense:
+        js0, js1 = elem_strides(cpl)
+        _lib.call("piqmc_quantum_anneal_dense", sched_c.ctypes.data, S, mcsteps, slices, temp,
+                  nspins, confs.ctypes.data, cs0, cs1, cpl.ctypes.data, js0, js1,
+                  perms.ctypes.data, st.ctypes.data, C.byref(draws), C.byref(ties))
+    else:
+        nbs_c = np.ascontiguousarray(cpl)
+        _lib.call("piqmc_quantum_anneal", sched_c.ctypes.data, S, mcsteps, slices, temp, nspins,
+                  confs.ctypes.data, cs0, cs1, nbs_c.ctypes.data, nbs_c.shape[1],
+                  perms.ctypes.data, st.ctypes.data, C.byref(draws), C.byref(ties))
     _crand.set_state(st)
     last_draws, last_near_ties = draws.value, ties.value
     return None

@@ -197,3 +197,48 @@ def test_colored_rejects_improper_coloring():
     sp = np.ones((4, 3), np.int8)
     with pytest.raises(ValueError):
         O.qa_colored(np.array([1.0]), 1, 4, 1.0, sp, nbs, np.array([0, 1, 1], np.int32), 2, 1)
+
+
+def _exact_classical_mean_energy(nbs, T):
+    N = nbs.shape[0]
+    S = np.array(list(itertools.product([1, -1], repeat=N)), np.int8)
+    E = O.energy_nbs(S, nbs)
+    w = np.exp(-(E - E.min()) / T)
+    return float((w * E).sum() / w.sum())
+
+
+def test_sa_colored_stationary_law_per_step_temperature():
+    # 64 chains of the frustrated triangle: sweeps at T=2.0, then at T=0.5.
+    # Each phase's energy average must match the exact Boltzmann mean at the
+    # temperature that phase's steps carry.
+    nbs, color, nc = _small_frustrated()
+    C, per_call, calls = 64, 10, 300
+    sp = np.ones((C, 3), np.int8)
+    for T in (2.0, 0.5):
+        exact = _exact_classical_mean_energy(nbs, T)
+        samples = []
+        for t in range(calls):
+            base = (0 if T == 2.0 else calls) * per_call + t * per_call
+            O.sa_colored(np.full(per_call, T), 1, sp, nbs, color, nc, seed=5, sweep_base=base)
+            if t >= 20:
+                samples.append(O.energy_nbs(sp, nbs).mean())
+        x = np.array(samples)
+        b = x[: len(x) // 20 * 20].reshape(20, -1).mean(1)
+        se = b.std(ddof=1) / np.sqrt(len(b))
+        assert abs(b.mean() - exact) < 5 * se + 2e-3, (T, b.mean(), exact, se)
+
+
+def test_sa_colored_is_the_parallel_sweep_with_zero_trotter_coupling():
+    # Gamma = 1e3 at P*T = 1: tanh = 1, J_perp = 0; chains c = slice c%P of replica c//P
+    nbs, color, nc = _small_frustrated()
+    C, T = 8, 0.7
+    rng = np.random.RandomState(1)
+    sp = np.where(rng.rand(C, 3) < 0.5, -1, 1).astype(np.int8)
+    a = sp.copy()
+    O.sa_colored(np.full(5, T), 2, a, nbs, color, nc, seed=9, replica_base=3, sweep_base=4)
+    b = sp.copy()
+    assert O.jperp(1e3 * T * C, C, T) == 0.0
+    # one replica of P=C slices; qa_colored's temp is the same T at every step
+    O.qa_colored(np.full(5, 1e3 * T * C), 2, C, T, b, nbs, color, nc, seed=9, replica=3,
+                 sweep_base=4)
+    assert np.array_equal(a, b)
