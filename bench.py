@@ -26,6 +26,26 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_ATTEMPT = 8  # int8 spin traffic of one 2-D attempt: self + 4 + 2 Trotter + write
 
 
+COUNTERS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "colored_pmj1024_counters.md")
+
+
+def measured_traffic(R, N, P, S):
+    """HBM bytes per launch of the headline kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a 1 GiB
+    device copy in the same run), when it covers this workload; else None."""
+    try:
+        with open(COUNTERS) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    _, val, key = line.split(None, 2)
+                    if key.strip() == "R=%d N=%d P=%d S=%d" % (R, N, P, S):
+                        return float(val)
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(nbs, N, P, T, sched, reps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -188,7 +208,7 @@ def main():
                                "Gamma 3.0->1e-8 over 500 steps x 1 sweep (parallel sweep)",
                    "replicas": R, "nspins": N, "slices": P, "field_steps": S, "sweeps_per_step": 1},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(R, N, P, S),
                      "kernel": "colored_kernel<uint32,+-1 path>", "tile": {"block": tile[0], "rpw": tile[1]},
                      "kernel_ms": kavg * 1e3, "kernel_attempts_per_s": k_rate,
                      "bytes_per_attempt": BYTES_PER_ATTEMPT,

@@ -135,6 +135,17 @@ int piqmc_anneal_dense(const double *sched, int S, int mcsteps, double *svec, in
                        const int64_t *perms, uint32_t *rand_state32, uint64_t *draws,
                        uint32_t *ties);
 
+/* sa.Anneal_multispin (piqmc/sa.pyx:282-405) for R packed sets of 64
+ * configurations: bits_dev uint64 [R][N] in/out (configuration k at bit 63-k,
+ * bit 1 = spin -1), G sweeps at temperatures temps_host[G] (float, one per
+ * sweep), perms_dev uint16 [G][R][N], U_dev float64 [G][N][R][64]: the
+ * uniform configuration k of set r compares against at attempt t of sweep g
+ * (the reference's rng.rand(64) stream, piqmc/sa.pyx:331,398).  ties_dev
+ * uint32 [R] is incremented by the near ties of the exp comparison. */
+int piqmc_sa_multispin(const piqmc_problem *pb, int R, int G, const float *temps_host,
+                       uint64_t *bits_dev, const uint16_t *perms_dev, const double *U_dev,
+                       uint32_t *ties_dev, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Parallel sweep ("fixed" semantics: periodic Trotter neighbours,     */
 /* per-spin reset) -- graph-colored in space, parity-alternated in     */

@@ -61,6 +61,10 @@ def lib():
                                      C.c_uint32, C.c_uint32, C.c_void_p]
         L.orc_sa_colored.argtypes = [f64p, C.c_int, C.c_int, C.c_int, C.c_int, i8p, f64p,
                                      C.c_int, i32p, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32]
+        L.orc_sa_multispin.argtypes = [np.ctypeslib.ndpointer(np.float32, flags="C"), C.c_int,
+                                       C.c_int, np.ctypeslib.ndpointer(np.uint64, flags="C"),
+                                       f64p, C.c_int, i64p, f64p]
+        L.orc_sa_multispin.restype = C.c_int
         L.orc_energy_nbs.argtypes = [i8p, C.c_int, C.c_int, f64p, C.c_int, f64p]
         L.orc_energy_coo.argtypes = [f64p, C.c_int, C.c_long, i64p, i64p, f64p]
         L.orc_energy_coo.restype = C.c_double
@@ -237,6 +241,39 @@ def sa_colored(sched, mcsteps, spins, nbs, color, ncolors, seed, replica_base=0,
                               int(seed) & 0xFFFFFFFFFFFFFFFF, int(replica_base), int(sweep_base))
     if rc:
         raise ValueError("orc_sa_colored rc=%d" % rc)
+
+
+def sa_multispin(temps, svec, nbs, perms, U):
+    """Anneal_multispin restatement on packed words svec uint64 [N] (in
+    place); temps float32 [G] per sweep; perms int64 [G, N]; U float64
+    [G, N, 64]."""
+    temps = np.ascontiguousarray(temps, np.float32)
+    nbs = np.ascontiguousarray(nbs, np.float64)
+    perms = np.ascontiguousarray(perms, np.int64)
+    U = np.ascontiguousarray(U, np.float64)
+    assert svec.dtype == np.uint64 and svec.flags.c_contiguous
+    G, N = len(temps), svec.shape[0]
+    assert perms.shape == (G, N) and U.shape == (G, N, 64)
+    rc = lib().orc_sa_multispin(temps, G, N, svec, nbs, nbs.shape[1], perms, U)
+    if rc:
+        raise ValueError("orc_sa_multispin rc=%d" % rc)
+
+
+def pack_multispin(bits):
+    """[64, N] 0/1 (nonzero = 1) -> uint64 [N], row k at bit 63-k
+    (piqmc/sa.pyx:336-343)."""
+    b = (np.asarray(bits) != 0).astype(np.uint64)
+    out = np.zeros(b.shape[1], np.uint64)
+    for k in range(b.shape[0]):
+        out = (out << np.uint64(1)) ^ b[k]
+    return out
+
+
+def unpack_multispin(words):
+    """uint64 [N] -> float64 [64, N] of 0/1 (piqmc/sa.pyx:402-405)."""
+    w = np.asarray(words, np.uint64)
+    return np.stack([((w >> np.uint64(63 - k)) & np.uint64(1)).astype(np.float64)
+                     for k in range(64)])
 
 
 def energy_nbs(spins, nbs):

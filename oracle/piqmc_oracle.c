@@ -29,6 +29,9 @@
  *                       with orc_qa_fixed_seq's per-attempt arithmetic.
  *   orc_energy_*      : ClassicalIsingEnergy known answers
  *                       (testing/test_boixo.py:60-70), instance ground states.
+ *   orc_sa_multispin  : each of the 64 packed configurations equals an
+ *                       independent +-1 Metropolis chain fed its own uniforms
+ *                       (tests/test_oracle_multispin.py); hand trace.
  *   orc_qa_parallel   : timing twin only (the reference is racy), unpinned.
  */
 #include <math.h>
@@ -370,6 +373,49 @@ int orc_sa_dense(const double *sched, int S, int M, double *svec, long ss, int N
         }
     }
     if (draws) *draws = nd;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* sa.Anneal_multispin, piqmc/sa.pyx:282-405: 64 configurations packed */
+/* in one uint64 per spin, configuration k at bit 63-k, bit 1 = spin   */
+/* -1 (:339-345).  Per attempt: ediffs[k] (double, reset per attempt   */
+/* :395-396) -= 2*jval where the configuration's bit of svec[sidx]     */
+/* (field) or of svec[sidx]^svec[spinidx] (coupling) is 0, += where it */
+/* is 1 (:355-379), jval = (float)nbs; flip configuration k iff        */
+/* exp(ediffs[k]/temp) > U[k] (:382-393), temp = (float)sched (:349).  */
+/* The uniforms come from the caller: U[(g*N + t)*64 + k] is what      */
+/* configuration k compares against at attempt t of sweep g (the rng   */
+/* stream order is reproduced by the caller).  perms[g*N + t].         */
+/* ------------------------------------------------------------------ */
+int orc_sa_multispin(const float *temps, int G, int N, uint64_t *svec, const double *nbs,
+                     int Z, const int64_t *perms, const double *U)
+{
+    if (N < 1 || Z < 1) return -1;
+    double ed[64];
+    for (int g = 0; g < G; ++g) {
+        const double temp = (double)temps[g];
+        for (int t = 0; t < N; ++t) {
+            const int sidx = (int)perms[(size_t)g * N + t];
+            for (int k = 0; k < 64; ++k) ed[k] = 0.0;
+            for (int si = 0; si < Z; ++si) {
+                const double *row = nbs + ((size_t)sidx * Z + si) * 2;
+                const int spinidx = (int)row[0];
+                const float jval = (float)row[1];
+                const uint64_t w = spinidx == sidx ? svec[sidx] : (svec[sidx] ^ svec[spinidx]);
+                for (int k = 0; k < 64; ++k) {
+                    if (!((w >> (63 - k)) & 1u)) ed[k] -= 2.0 * (double)jval;
+                    else ed[k] += 2.0 * (double)jval;
+                }
+            }
+            uint64_t fm = 0;
+            for (int k = 0; k < 64; ++k) {
+                fm <<= 1;
+                if (exp(ed[k] / temp) > U[((size_t)g * N + t) * 64 + k]) fm ^= 1u;
+            }
+            svec[sidx] ^= fm;
+        }
+    }
     return 0;
 }
 

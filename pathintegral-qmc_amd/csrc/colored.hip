@@ -134,6 +134,10 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
         __syncthreads();
         // uniform-degree lattices: the class thresholds of this step are wave-uniform
         const uint32_t zt_u = (PMJ && UZ) ? __builtin_amdgcn_readfirstlane(a.ztab[s * 8 + a.zuni]) : 0u;
+        // its six thresholds (accept / stochastic per Trotter class), clamped to the table
+        const int ka0 = min((int)(zt_u & 15), 8), ks0 = min((int)((zt_u >> 4) & 15), 8);
+        const int ka1 = min((int)((zt_u >> 8) & 15), 8), ks1 = min((int)((zt_u >> 12) & 15), 8);
+        const int ka2 = min((int)((zt_u >> 16) & 15), 8), ks2 = min((int)((zt_u >> 20) & 15), 8);
         const float jp = a.jperp[s];
         const double Ts = (double)a.temps[s];
         for (int m = 0; m < a.M; ++m) {
@@ -177,12 +181,30 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                     }
                                 }
                                 const WT e0 = ~(uL | uR), e1 = uL ^ uR, e2 = uL & uR;
-                                flips = (e0 & ge_pick((int)(zt & 15), c0, c1, c2)) |
-                                        (e1 & ge_pick((int)((zt >> 8) & 15), c0, c1, c2)) |
-                                        (e2 & ge_pick((int)((zt >> 16) & 15), c0, c1, c2));
-                                WT X = (e0 & ge_pick((int)((zt >> 4) & 15), c0, c1, c2)) |
-                                       (e1 & ge_pick((int)((zt >> 12) & 15), c0, c1, c2)) |
-                                       (e2 & ge_pick((int)((zt >> 20) & 15), c0, c1, c2));
+                                WT X;
+                                if (UZ) {
+                                    // thresholds are wave-uniform: every "count >= k" mask
+                                    // once, then six selections by a uniform index
+                                    WT ge[9];
+                                    ge[0] = ~WT(0);
+                                    ge[1] = c0 | c1 | c2;
+                                    ge[2] = c1 | c2;
+                                    ge[3] = c2 | (c1 & c0);
+                                    ge[4] = c2;
+                                    ge[5] = c2 & (c1 | c0);
+                                    ge[6] = c2 & c1;
+                                    ge[7] = c2 & c1 & c0;
+                                    ge[8] = WT(0);
+                                    flips = (e0 & ge[ka0]) | (e1 & ge[ka1]) | (e2 & ge[ka2]);
+                                    X = (e0 & ge[ks0]) | (e1 & ge[ks1]) | (e2 & ge[ks2]);
+                                } else {
+                                    flips = (e0 & ge_pick((int)(zt & 15), c0, c1, c2)) |
+                                            (e1 & ge_pick((int)((zt >> 8) & 15), c0, c1, c2)) |
+                                            (e2 & ge_pick((int)((zt >> 16) & 15), c0, c1, c2));
+                                    X = (e0 & ge_pick((int)((zt >> 4) & 15), c0, c1, c2)) |
+                                        (e1 & ge_pick((int)((zt >> 12) & 15), c0, c1, c2)) |
+                                        (e2 & ge_pick((int)((zt >> 20) & 15), c0, c1, c2));
+                                }
                                 flips &= valid;
                                 X &= valid & ~flips;
                                 if (X) {

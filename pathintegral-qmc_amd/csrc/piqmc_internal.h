@@ -14,6 +14,9 @@
 namespace piqmc {
 int set_error(int code, const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
+// PIQMC_EINVAL (with a message) if any of the n uint16 permutation entries on
+// the device is >= N; synchronizes the stream.
+int check_perms(const uint16_t *perms, size_t n, int N, hipStream_t st);
 }  // namespace piqmc
 
 #define PIQMC_HIP(call)                                                   \

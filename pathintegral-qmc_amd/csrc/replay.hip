@@ -12,7 +12,8 @@
 //   - slices 1 and P-1, the only Trotter neighbours the reference reads (D1),
 //   - the 31-word glibc TYPE_3 ring of its rand() stream.
 // The permutation stream is read from HBM as [sweep][position][replica]
-// uint16, eight attempts ahead of use so its latency leaves the chain.
+// uint16; each replica's permutation of a sweep is staged in LDS once and
+// read by every slice of the sweep, one attempt ahead of use.
 //
 // Arithmetic is the reference's: ediff is float and every term is an exact
 // +-2*float product, so the float add equals the reference's double add
@@ -27,7 +28,7 @@
 namespace {
 
 struct ReplayArgs {
-    int R, N, P, Z, S, M, Wn, sa, rows_lds, Lw;
+    int R, N, P, Z, Zp, S, M, Wn, sa, Lw;
     float temp;
     const float *steps;     // [S] J_perp (QA) or float temperature (SA)
     const int2 *rows;       // [N][Z] (j, J bits)
@@ -44,7 +45,16 @@ __device__ __forceinline__ uint32_t getbit(const uint32_t *reg, int L, int lane,
     return (reg[(i >> 5) * L + lane] >> (i & 31)) & 1u;
 }
 
-template <bool F64J>
+// LDS image of a workgroup with L replica slots, words slot-interleaved:
+//   spin regions [nreg][Wn][L] u32, rand() rings [31][L] u32, and with STAGED
+//   (8-byte aligned) the neighbour rows [N][Zp] int2, padded with zero fields
+//   to a multiple of 4 entries, then the sweep's permutations [N][L] u16.
+__host__ __device__ inline size_t replay_words(int nreg, int Wn, int L)
+{
+    return ((size_t)(nreg * Wn + 31) * L + 1) & ~(size_t)1;
+}
+
+template <bool F64J, bool STAGED>
 __global__ void replay_kernel(ReplayArgs a)
 {
     extern __shared__ uint32_t sm[];
@@ -52,19 +62,26 @@ __global__ void replay_kernel(ReplayArgs a)
     // replica's sweep is a dependency chain, so few lanes per wave spread the
     // replicas over every SIMD instead of stacking them in a few waves.
     const int Lw = a.Lw, L = (blockDim.x >> 6) * Lw;  // L: slots per workgroup (LDS word stride)
+    const int lsh = 31 - __clz(L);                    // L is a power of two
     const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
     const int lane = wv * Lw + ln;
     const int r = blockIdx.x * L + lane;
     const int N = a.N, P = a.P, Z = a.Z, Wn = a.Wn, R = a.R;
     const int nreg = a.sa ? 1 : (P == 2 ? 2 : 3);
-    // neighbour rows staged in LDS (every thread helps, then the only barrier)
-    const int2 *rows = a.rows;
-    if (a.rows_lds) {
-        // rows start on an 8-byte boundary after the spin words and the rand() rings
-        int2 *lr = reinterpret_cast<int2 *>(sm + (((size_t)(nreg * Wn + 31) * L + 1) & ~(size_t)1));
-        for (int q = threadIdx.x; q < N * Z; q += blockDim.x) lr[q] = a.rows[q];
+    const int Zr = STAGED ? a.Zp : Z;  // row stride of the table in use
+    int2 *lrows = reinterpret_cast<int2 *>(sm + replay_words(nreg, Wn, L));
+    uint16_t *lperm = reinterpret_cast<uint16_t *>(lrows + (size_t)N * a.Zp);
+    const int2 *rows = STAGED ? lrows : a.rows;
+    if (STAGED) {  // every thread helps, then the only barrier
+        // float couplings are staged as -2J (exact): the term -2 s_i s_j J is
+        // then -2J with its sign bit flipped when the two spins differ
+        for (int q = threadIdx.x; q < N * a.Zp; q += blockDim.x) {
+            const int i = q / a.Zp, z = q - i * a.Zp;
+            int2 v = z < Z ? a.rows[(size_t)i * Z + z] : make_int2(i, 0);
+            if (!F64J) v.y = __float_as_int(-2.0f * __int_as_float(v.y));
+            lrows[q] = v;
+        }
         __syncthreads();
-        rows = lr;
     }
     if (ln >= Lw || r >= a.R) return;  // no barriers below: a slot touches only its own words
     uint32_t *cur = sm, *s1, *sp, *ring;
@@ -94,87 +111,130 @@ __global__ void replay_kernel(ReplayArgs a)
     uint32_t nt = 0;
     const float T = a.temp;
     const int nslice = a.sa ? 1 : P;
-    constexpr int kAhead = 8;  // permutation entries loaded one block ahead
 
     for (int s = 0; s < a.S; ++s) {
         const float sv = a.steps[s];
         const float tdiv = a.sa ? sv : T;
+        const uint32_t svm2 = __float_as_uint(-2.0f * sv);  // exact
         for (int m = 0; m < a.M; ++m) {
             const uint16_t *pg = a.perms + (size_t)(s * a.M + m) * N * R + r;
+            if (STAGED) {
+                // this replica's permutation, used by every slice of the
+                // sweep, into its LDS column: loads in groups of 8, clamped so
+                // that none is conditional
+                for (int t0 = 0; t0 < N; t0 += 8) {
+                    uint16_t v[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) v[q] = pg[(size_t)min(t0 + q, N - 1) * R];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (t0 + q < N) lperm[(size_t)(t0 + q) * L + lane] = v[q];
+                }
+            }
             for (int k = 0; k < nslice; ++k) {
                 uint32_t *reg = a.sa ? cur : (k == 1 ? s1 : (k == P - 1 ? sp : cur));
                 const bool own = !a.sa && reg == cur;
                 if (own) load(cur, k);
                 float e = 0.0f;
-                int blk[kAhead];
+                int inext = STAGED ? (int)lperm[lane] : (int)pg[0];
+                for (int t = 0; t < N; ++t) {
+                    const int i = inext;  // the next position's index is read one attempt ahead
+                    const int tn = min(t + 1, N - 1);
+                    inext = STAGED ? (int)lperm[((uint32_t)tn << lsh) + lane] : (int)pg[(size_t)tn * R];
+                    // every LDS word this attempt needs that does not depend
+                    // on another read is read first, together
+                    const uint32_t wo = ((uint32_t)i >> 5) << lsh;
+                    uint32_t *wself = reg + wo + lane;
+                    const uint32_t ws = *wself;
+                    const uint32_t wl = a.sa ? 0u : sp[wo + lane];  // slice P-1
+                    const uint32_t wr = a.sa ? 0u : s1[wo + lane];  // slice 1
+                    const uint32_t rpi = f >= 3 ? f - 3 : f + 28;
+                    const uint32_t rf = ring[(f << lsh) + lane], rr = ring[(rpi << lsh) + lane];
+                    const int2 *row = rows + __umul24((uint32_t)i, (uint32_t)Zr);
+                    int2 rw[4];
+                    if constexpr (STAGED && !F64J) {
 #pragma unroll
-                for (int j = 0; j < kAhead; ++j) blk[j] = j < N ? pg[(size_t)j * R] : 0;
-                for (int t0 = 0; t0 < N; t0 += kAhead) {
-                    int nxt[kAhead];
-#pragma unroll
-                    for (int j = 0; j < kAhead; ++j) {
-                        const int t = t0 + kAhead + j;
-                        nxt[j] = t < N ? pg[(size_t)t * R] : 0;
+                        for (int q = 0; q < 4; ++q) rw[q] = row[q];
                     }
+                    __builtin_amdgcn_sched_barrier(0);
+                    const uint32_t bsh = (uint32_t)i & 31u;
+                    const uint32_t bi = 1u << bsh;
+                    const uint32_t xi = (ws >> bsh) & 1u;  // 1: s_i = -1
+                    const float m2 = xi ? 2.0f : -2.0f;    // -2*s_i
+                    if (a.sa) e = 0.0f;
+                    if constexpr (F64J) {
+                        // *_dense: J is read as double, so each term is
+                        // added in double and the sum rounded to float
+                        // (piqmc/qmc.pyx:209-218, piqmc/sa.pyx:172-179)
+                        const double *jrow = a.jd + (size_t)i * Z;
+                        for (int z = 0; z < Z; ++z) {
+                            const int j = row[z].x;
+                            const double J = jrow[z];
+                            const double t2 = j == i ? (double)m2 * J
+                                                     : (double)m2 * (getbit(reg, L, lane, j) ? -J : J);
+                            e = (float)((double)e + t2);
+                        }
+                    } else if constexpr (STAGED) {
+                        // four entries at a time: the spin reads of a group
+                        // issue together; the adds keep row order.  Entry
+                        // (j, -2J): the term is -2J, sign flipped when s_i
+                        // and s_j differ (j == i: when s_i = -1)
+                        for (int z = 0; z < Zr; z += 4) {
+                            if (z) {
 #pragma unroll
-                    for (int j = 0; j < kAhead; ++j) {
-                        if (t0 + j >= N) break;
-                        const int i = blk[j];
-                        const float m2 = getbit(reg, L, lane, i) ? 2.0f : -2.0f;  // -2*s_i
-                        if (a.sa) e = 0.0f;
-                        const int2 *row = rows + (size_t)i * Z;
-                        if constexpr (F64J) {
-                            // *_dense: J is read as double, so each term is
-                            // added in double and the sum rounded to float
-                            // (piqmc/qmc.pyx:209-218, piqmc/sa.pyx:172-179)
-                            const double *jrow = a.jd + (size_t)i * Z;
-                            for (int z = 0; z < Z; ++z) {
-                                const int j = row[z].x;
-                                const double J = jrow[z];
-                                const double t = j == i ? (double)m2 * J
-                                                        : (double)m2 * (getbit(reg, L, lane, j) ? -J : J);
-                                e = (float)((double)e + t);
+                                for (int q = 0; q < 4; ++q) rw[q] = row[z + q];
                             }
-                        } else {
-                            for (int z = 0; z < Z; ++z) {
-                                const int2 rw = row[z];
-                                const float J = __int_as_float(rw.y);
-                                if (rw.x == i) {
-                                    e = e + m2 * J;
-                                } else {
-                                    const float sj = getbit(reg, L, lane, rw.x) ? -1.0f : 1.0f;
-                                    e = e + m2 * (J * sj);
-                                }
+                            uint32_t wn[4];
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) wn[q] = reg[(((uint32_t)rw[q].x >> 5) << lsh) + lane];
+                            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) {
+                                const uint32_t bj = rw[q].x == i ? 0u : (wn[q] >> (rw[q].x & 31)) & 1u;
+                                e = e + __uint_as_float((uint32_t)rw[q].y ^ ((xi ^ bj) << 31));
                             }
                         }
-                        bool flip;
-                        if (!a.sa) {
-                            const float sl = getbit(sp, L, lane, i) ? -1.0f : 1.0f;  // slice P-1
-                            e = e + m2 * (sv * sl);
-                            const float sr = getbit(s1, L, lane, i) ? -1.0f : 1.0f;  // slice 1
-                            e = e + m2 * (sv * sr);
-                            flip = e > 0.0f;
-                        } else {
-                            // sa.Anneal accepts ediff >= 0 (piqmc/sa.pyx:114),
-                            // Anneal_dense ediff > 0 (piqmc/sa.pyx:181)
-                            flip = F64J ? e > 0.0f : e >= 0.0f;
+                    } else {
+                        for (int z = 0; z < Z; ++z) {
+                            const int2 r1 = row[z];
+                            const float J = __int_as_float(r1.y);
+                            if (r1.x == i) {
+                                e = e + m2 * J;
+                            } else {
+                                const float sj = getbit(reg, L, lane, r1.x) ? -1.0f : 1.0f;
+                                e = e + m2 * (J * sj);
+                            }
                         }
-                        if (!flip) {
-                            const uint32_t rp = f >= 3 ? f - 3 : f + 28;
-                            const uint32_t v = ring[f * L + lane] + ring[rp * L + lane];
-                            ring[f * L + lane] = v;
-                            f = f == 30 ? 0 : f + 1;
-                            ++nd;
-                            const double u = (double)(v >> 1) / 2147483647.0;
+                    }
+                    bool flip;
+                    if (!a.sa) {
+                        // -2 s_i (J_perp s_l) = -2 J_perp, sign flipped when s_i != s_l
+                        e = e + __uint_as_float(svm2 ^ ((xi ^ ((wl >> bsh) & 1u)) << 31));
+                        e = e + __uint_as_float(svm2 ^ ((xi ^ ((wr >> bsh) & 1u)) << 31));
+                        flip = e > 0.0f;
+                    } else {
+                        // sa.Anneal accepts ediff >= 0 (piqmc/sa.pyx:114),
+                        // Anneal_dense ediff > 0 (piqmc/sa.pyx:181)
+                        flip = F64J ? e > 0.0f : e >= 0.0f;
+                    }
+                    if (!flip) {
+                        const uint32_t v = rf + rr;
+                        ring[(f << lsh) + lane] = v;
+                        f = f == 30 ? 0 : f + 1;
+                        ++nd;
+                        const uint32_t x = v >> 1;
+                        // exp(e/T) <= exp(-22) < 2.8e-10 < 1/2147483647 <= u
+                        // whenever e < -22.5 T (T > 0, x > 0): rejected without exp
+                        const bool far = tdiv > 0.0f && tdiv < INFINITY && e < -22.5f * tdiv && x != 0u;
+                        if (!far) {
+                            const double u = (double)x / 2147483647.0;
                             const float q = e / tdiv;
                             const double ex = exp((double)q);
                             flip = ex > u;
                             if (ex > 0.0 && ex < INFINITY && fabs(ex - u) <= ex * 0x1p-50) ++nt;
                         }
-                        if (flip) reg[(i >> 5) * L + lane] ^= 1u << (i & 31);
                     }
-#pragma unroll
-                    for (int j = 0; j < kAhead; ++j) blk[j] = nxt[j];
+                    if (flip) *wself = ws ^ bi;
                 }
                 if (own) store(cur, k);
             }
@@ -248,35 +308,39 @@ int run_replay(const piqmc_problem *pb, int R, int P, int S, int M, float temp, 
     // Launch shape: W waves per workgroup share one LDS copy of the neighbour
     // rows; Lw lanes per wave each own a replica.  Lw is the smallest power of
     // two that still gives every CU a workgroup, so replicas spread over all
-    // SIMDs; rows move to global memory only when they cannot fit.
-    const size_t rows_bytes = (size_t)N * pb->Z * sizeof(int2);
-    auto lds_of = [&](int slots, bool rows) {
-        const size_t words = (size_t)(nreg * Wn + 31) * slots;
-        return rows ? ((words + 1) & ~(size_t)1) * 4 + rows_bytes : words * 4;
+    // SIMDs.  The staged form (rows and the sweep's permutations in LDS) is
+    // preferred; shapes shrink until it fits, else rows and permutations are
+    // read from global memory.
+    const int Zp = (pb->Z + 3) & ~3;
+    auto lds_of = [&](int slots, bool staged) {
+        const size_t words = replay_words(nreg, Wn, slots) * 4;
+        return staged ? words + (size_t)N * Zp * sizeof(int2) + (size_t)N * slots * 2 : words;
     };
     constexpr size_t kLds = 160 * 1024;
     constexpr int kCUs = 256;
-    int W = 4, Lw = 64;
-    while (Lw > 1 && (size_t)W * Lw * kCUs > (size_t)R) Lw /= 2;
-    bool rows_lds = lds_of(W * Lw, true) <= kLds;
-    while (lds_of(W * Lw, rows_lds) > kLds && (W > 1 || Lw > 1)) {
+    int W0 = 4, Lw0 = 64;
+    while (Lw0 > 1 && (size_t)W0 * Lw0 * kCUs > (size_t)R) Lw0 /= 2;
+    int W = W0, Lw = Lw0;
+    bool staged = true;
+    while (lds_of(W * Lw, true) > kLds && (W > 1 || Lw > 1)) {
         if (Lw > 1) Lw /= 2; else W /= 2;
-        if (!rows_lds && lds_of(W * Lw, true) <= kLds) rows_lds = true;
+    }
+    if (lds_of(W * Lw, true) > kLds) {
+        staged = false;
+        W = W0;
+        Lw = Lw0;
+        while (lds_of(W * Lw, false) > kLds && (W > 1 || Lw > 1)) {
+            if (Lw > 1) Lw /= 2; else W /= 2;
+        }
     }
     const int L = W * Lw;
-    if (lds_of(L, rows_lds) > kLds)
+    if (lds_of(L, staged) > kLds)
         return piqmc::set_error(PIQMC_EINVAL, "replay: nspins too large for one workgroup's LDS");
-    auto lds = [&](int l) { return lds_of(l, rows_lds); };
+    auto lds = [&](int l) { return lds_of(l, staged); };
     const size_t nperm = (size_t)S * M * N * R;
     if (nperm && N < 65536) {
-        int *d_bad = nullptr, bad = 0;
-        PIQMC_HIP(hipMallocAsync((void **)&d_bad, sizeof(int), st));
-        PIQMC_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), st));
-        check_perms_kernel<<<grid_for(nperm, 256), 256, 0, st>>>(perms, nperm, N, d_bad);
-        PIQMC_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st));
-        PIQMC_HIP(hipFreeAsync(d_bad, st));
-        PIQMC_HIP(hipStreamSynchronize(st));
-        if (bad) return piqmc::set_error(PIQMC_EINVAL, "permutation entry out of range [0, nspins)");
+        const int rc = piqmc::check_perms(perms, nperm, N, st);
+        if (rc) return rc;
     }
     float *d_steps = nullptr;
     uint32_t *d_bits = nullptr;
@@ -287,7 +351,7 @@ int run_replay(const piqmc_problem *pb, int R, int P, int S, int M, float temp, 
     pack_lanes_kernel<<<grid_for((size_t)R * P * Wn, 256), 256, 0, st>>>(confs, d_bits, R, P, N, Wn);
     ReplayArgs a;
     a.R = R; a.N = N; a.P = P; a.Z = pb->Z; a.S = S; a.M = M; a.Wn = Wn; a.sa = sa;
-    a.rows_lds = rows_lds ? 1 : 0;
+    a.Zp = Zp;
     a.Lw = Lw;
     a.temp = temp;
     a.steps = d_steps;
@@ -298,7 +362,8 @@ int run_replay(const piqmc_problem *pb, int R, int P, int S, int M, float temp, 
     a.draws = d_draws;
     a.ties = ties;
     a.jd = pb->dense ? pb->d_jd : nullptr;
-    auto kern = pb->dense ? replay_kernel<true> : replay_kernel<false>;
+    auto kern = pb->dense ? (staged ? replay_kernel<true, true> : replay_kernel<true, false>)
+                          : (staged ? replay_kernel<false, true> : replay_kernel<false, false>);
     (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds(L));
     piqmc::timing_begin(st);
     kern<<<(R + L - 1) / L, 64 * W, lds(L), st>>>(a);
@@ -312,6 +377,21 @@ int run_replay(const piqmc_problem *pb, int R, int P, int S, int M, float temp, 
 }
 
 }  // namespace
+
+namespace piqmc {
+int check_perms(const uint16_t *perms, size_t n, int N, hipStream_t st)
+{
+    int *d_bad = nullptr, bad = 0;
+    PIQMC_HIP(hipMallocAsync((void **)&d_bad, sizeof(int), st));
+    PIQMC_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), st));
+    check_perms_kernel<<<grid_for(n, 256), 256, 0, st>>>(perms, n, N, d_bad);
+    PIQMC_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st));
+    PIQMC_HIP(hipFreeAsync(d_bad, st));
+    PIQMC_HIP(hipStreamSynchronize(st));
+    if (bad) return set_error(PIQMC_EINVAL, "permutation entry out of range [0, nspins)");
+    return PIQMC_OK;
+}
+}  // namespace piqmc
 
 extern "C" int piqmc_qa_replay(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
                                const double *sched, int8_t *confs, const uint16_t *perms,

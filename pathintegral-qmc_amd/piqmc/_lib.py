@@ -43,6 +43,7 @@ _SIGS = {
     "piqmc_quantum_anneal_dense": ([_vp, _i, _i, _i, _f, _i, _vp, _i64, _i64, _vp, _i64, _i64,
                                     _vp, _vp, _vp, _vp], _i),
     "piqmc_anneal_dense": ([_vp, _i, _i, _vp, _i64, _i, _vp, _i64, _i64, _vp, _vp, _vp, _vp], _i),
+    "piqmc_sa_multispin": ([_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "piqmc_sa_colored": ([_vp, _i, _i, _i, _vp, _vp, _u64, _u32, _u32, _i, _i, _vp], _i),
     "piqmc_qa_colored": ([_vp, _i, _i, _i, _i, _f, _vp, _vp, _u64, _u32, _u32, _vp, _i, _i, _vp],
                          _i),

@@ -4,7 +4,8 @@
 acceptance, float temperature per step, piqmc/sa.pyx:50-120) and reproduces
 the Cython code bit-for-bit, drawing the same permutations from ``rng`` and
 the same libc ``rand()`` stream.  ``Anneal_dense`` (piqmc/sa.pyx:126-187) is
-the same for a dense coupling matrix.  ``ClassicalIsingEnergy`` evaluates
+the same for a dense coupling matrix.  ``Anneal_multispin`` (piqmc/sa.pyx:282-405)
+runs 64 packed configurations per wavefront.  ``ClassicalIsingEnergy`` evaluates
 piqmc/sa.pyx:41-44 on the GPU.
 """
 import ctypes as C
@@ -108,4 +109,78 @@ def _anneal(sched, mcsteps, svec, cpl, rng, dense):
                   perms.ctypes.data, st.ctypes.data, C.byref(draws), C.byref(ties))
     _crand.set_state(st)
     last_draws, last_near_ties = draws.value, ties.value
+    return None
+
+
+#: uniforms per chunk of sweeps sent to the device at once (float64 count)
+_MULTISPIN_CHUNK = 1 << 24
+
+
+def Anneal_multispin(sched, mcsteps, svec_mat, nbs, rng):
+    """Anneal_multispin(sched, mcsteps, svec_mat, nbs, rng)
+
+    64 simultaneous thermal anneals, piqmc/sa.pyx:282-405.  ``svec_mat`` is
+    float64 [64, nspins] of bits (nonzero = 1 = spin -1) and is overwritten
+    with the final bits (0.0 / 1.0).  ``rng`` is consumed exactly as the
+    reference consumes it: rand(64), permutation(range(nspins)), then per
+    attempt rand(64) and per sweep permutation.  Returns None.
+
+    Deviation: the reference's unpack loop runs to nspins inclusive
+    (piqmc/sa.pyx:402) and reads/writes one element past the arrays; only the
+    nspins real columns are written here.  svec_mat must have 64 rows (the
+    reference's unpack writes 64 rows whatever the row count).
+    """
+    global last_near_ties
+    import torch
+    from . import device
+
+    sched = f64_view(sched, 1, "sched")
+    svec_mat = f64_view(svec_mat, 2, "svec_mat", writable=True)
+    nbs = f64_view(nbs, 3, "nbs")
+    mcsteps = c_int(mcsteps)
+    nspins = svec_mat.shape[1]
+    rands = rng.rand(64)                                   # piqmc/sa.pyx:331
+    perm = rng.permutation(range(nspins))                  # :336-337
+    if svec_mat.shape[0] != 64:
+        raise ValueError("svec_mat must have 64 rows (one per packed configuration), got %d"
+                         % svec_mat.shape[0])
+    if nspins == 0:
+        return None
+    if nbs.shape[0] != nspins:
+        raise ValueError("nbs must have one row per spin")
+    bits = (svec_mat != 0)
+    words = np.zeros(nspins, np.uint64)                     # :339-345
+    for k in range(64):
+        words = (words << np.uint64(1)) ^ bits[k].astype(np.uint64)
+    S, M = sched.shape[0], max(mcsteps, 0)
+    last_near_ties = 0
+    if S * M > 0:
+        pb = device.Problem(nbs)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        d_bits = torch.from_numpy(words.view(np.int64).copy()).to(dev)
+        d_ties = torch.zeros(1, dtype=torch.int32, device=dev)
+        temps = np.repeat(sched.astype(np.float32), M)    # :349, temp = (float)sched
+        per = max(1, _MULTISPIN_CHUNK // (64 * nspins))    # sweeps per device call
+        for g0 in range(0, S * M, per):
+            G = min(per, S * M - g0)
+            perms = np.empty((G, nspins), np.uint16)
+            U = np.empty((G, nspins, 64))
+            for g in range(G):
+                perms[g] = perm
+                d = rng.rand(64 * nspins).reshape(nspins, 64)   # :398, one rand(64) per attempt
+                U[g, 0] = rands
+                U[g, 1:] = d[:-1]
+                rands = d[-1]
+                perm = rng.permutation(perm)                    # :400
+            t_chunk = np.ascontiguousarray(temps[g0:g0 + G])
+            d_perms = torch.from_numpy(perms.view(np.int16)).to(dev)
+            d_U = torch.from_numpy(U).to(dev)
+            _lib.call("piqmc_sa_multispin", pb._h, 1, G, t_chunk.ctypes.data, d_bits.data_ptr(),
+                      d_perms.data_ptr(), d_U.data_ptr(), d_ties.data_ptr(),
+                      C.c_void_p(torch.cuda.current_stream().cuda_stream))
+            torch.cuda.synchronize()
+        words = d_bits.cpu().numpy().view(np.uint64)
+        last_near_ties = int(d_ties.item())
+    for k in range(64):                                    # :402-405
+        svec_mat[k, :] = ((words >> np.uint64(63 - k)) & np.uint64(1)).astype(np.float64)
     return None
