@@ -118,6 +118,33 @@ def batch_leg(n_inst=512, reps=4, seed=0):
                         "P in {16,32,64,128}, S ~ U[100,1000], one persistent launch"}
 
 
+def c5_leg(steps=2000, replicas=1024):
+    """3-D +-J glass L=12 (N=1728) periodic with fields h_i ~ N(0, 0.1^2)
+    (float32), P=128, T=1/128, Gamma 3.0 -> 1e-8: one parallel-sweep anneal."""
+    import numpy as np
+    import torch
+    from piqmc import device, instances
+    N, r, c, v = instances.lattice_pm_j(12, 3, np.random.RandomState(5), field_sigma=0.1)
+    pb = device.Problem(instances.neighbors(N, r, c, v))
+    P, T = 128, 1.0 / 128
+    sched = np.linspace(3.0, 1e-8, steps)
+    g = torch.Generator(device="cpu").manual_seed(1)
+    confs = (torch.randint(0, 2, (replicas, P, N), generator=g, dtype=torch.int8) * 2 - 1).cuda()
+    pb.qa_colored(confs[:8].clone(), sched[:4], 1, T, 3)          # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pb.qa_colored(confs, sched, 1, T, 3)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    from piqmc import device as D
+    E = pb.slice_energies(confs)
+    return {"attempts_per_s": N * P * steps * replicas / dt, "seconds": dt,
+            "kernel_ms": D.last_kernel_ms(), "pm_j_path": pb.pm_j,
+            "mean_min_slice_energy_per_spin": float(E.min(1).values.mean()) / N,
+            "workload": "3-D +-J L=12 N=1728 periodic, h~N(0,0.1^2), P=128, T=1/128, "
+                        "%d replicas, Gamma 3.0->1e-8 over %d steps (parallel sweep)" % (replicas, steps)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +160,8 @@ def main():
     ap.add_argument("--replay-steps", type=int, default=25)
     ap.add_argument("--no-batch", action="store_true", help="skip the heterogeneous batch leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
+    ap.add_argument("--c5", action="store_true",
+                    help="also run the 3-D L=12 field workload (P=128, 1024 replicas, 2000 steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the sweep shards replicas with no exchange, so "
@@ -242,6 +271,8 @@ def main():
 
     if not args.no_batch:
         out["batch"] = batch_leg()
+    if args.c5:
+        out["c5"] = c5_leg()
     if not args.no_learned:
         from piqmc import learned
         lr = learned.run(eval_steps=(25, 50, 100, 200))

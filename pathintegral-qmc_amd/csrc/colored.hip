@@ -375,7 +375,7 @@ __device__ void dense_replica(const ColArgs<WT> &a, int r, WT *st)
                 WT *mine = st + tau * N;
                 const WT *other = st + (1 - tau) * N;
                 for (int q = 0; q < N; ++q) {  // colors in order; spins of one color are independent
-                    const int i = a.order[q];
+                    const int i = __builtin_amdgcn_readfirstlane(a.order[q]);  // wave-uniform
                     const WT S_ = mine[i], O = other[i];
                     const int lb = lane < W ? lane : 0;
                     // Trotter neighbours of slice 2lb+tau: slices 2lb+tau-1 and 2lb+tau+1
@@ -383,12 +383,37 @@ __device__ void dense_replica(const ColArgs<WT> &a, int r, WT *st)
                     const int mr = tau == 0 ? lb : (lb == W - 1 ? 0 : lb + 1);
                     const double si = ((S_ >> lb) & one) ? -1.0 : 1.0;
                     double h = 0.0;
-                    const int2 *row = a.rows + (size_t)i * a.Z;
-                    for (int e = 0; e < a.Z; ++e) {
+                    const int2 *row = a.rows + (size_t)i * a.Z;  // uniform: scalar loads
+                    int e = 0;
+                    // eight entries at a time: the row reads of the next group
+                    // are issued (clamped, so unconditionally) before the
+                    // spin-word reads and adds of this one; the adds keep row
+                    // order (J*(+-1) is exact, so the select form adds the same values)
+                    if (a.Z >= 8) {
+                        int2 rw[8];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) rw[k] = row[k];
+                        for (; e + 8 <= a.Z; e += 8) {
+                            int2 nx[8];
+                            const int en = min(e + 8, a.Z - 8);
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) nx[k] = row[en + k];
+                            WT wj[8];
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) wj[k] = mine[rw[k].x];
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) {
+                                const double J = (double)__int_as_float(rw[k].y);
+                                h += rw[k].x == i ? J : (((wj[k] >> lb) & one) ? -J : J);
+                            }
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) rw[k] = nx[k];
+                        }
+                    }
+                    for (; e < a.Z; ++e) {
                         const int2 rw = row[e];
                         const double J = (double)__int_as_float(rw.y);
-                        if (rw.x == i) h += J;
-                        else h += J * (((mine[rw.x] >> lb) & one) ? -1.0 : 1.0);
+                        h += rw.x == i ? J : (((mine[rw.x] >> lb) & one) ? -J : J);
                     }
                     const double ts = (((O >> ml) & one) ? -1.0 : 1.0) + (((O >> mr) & one) ? -1.0 : 1.0);
                     const double ed = -2.0 * si * (h + (double)jp * ts);
