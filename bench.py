@@ -139,7 +139,8 @@ def c5_leg(steps=2000, replicas=1024):
     from piqmc import device as D
     E = pb.slice_energies(confs)
     return {"attempts_per_s": N * P * steps * replicas / dt, "seconds": dt,
-            "kernel_ms": D.last_kernel_ms(), "pm_j_path": pb.pm_j,
+            "kernel_ms": D.last_kernel_ms(),
+            "path": "+-1 with fields" if pb.pm_j_fields else ("+-1" if pb.pm_j else "general"),
             "mean_min_slice_energy_per_spin": float(E.min(1).values.mean()) / N,
             "workload": "3-D +-J L=12 N=1728 periodic, h~N(0,0.1^2), P=128, T=1/128, "
                         "%d replicas, Gamma 3.0->1e-8 over %d steps (parallel sweep)" % (replicas, steps)}

@@ -61,7 +61,9 @@ int piqmc_problem_create_dense(int nspins, const double *J_host, int64_t s0, int
                                piqmc_problem **out);
 int piqmc_problem_destroy(piqmc_problem *pb);
 /* ncolors: spatial colors found; pm_j: 1 if every coupling is +-1 between
- * distinct spins (bit-sliced path), zmax: max couplings per spin. */
+ * distinct spins (bit-sliced path), 2 if in addition spins carry fields with
+ * 2^-27 <= |h| <= 2^20 (bit-sliced path with per-spin thresholds), else 0;
+ * zmax: max couplings per spin. */
 int piqmc_problem_info(const piqmc_problem *pb, int *nspins, int *maxnb,
                        int *ncolors, int *pm_j, int *zmax);
 /* Replace the spatial coloring (color_host[nspins], values 0..ncolors-1).

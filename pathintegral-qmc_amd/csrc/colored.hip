@@ -40,6 +40,7 @@ struct ColArgs {
     const int *color_off;      // [ncolors+1]
     const int32_t *cnb;        // [zmax][N] (+-1 path)
     const uint32_t *cinfo;     // [N]
+    const float *hq;           // [N] field of spin order[q] (+-1 couplings with fields)
     const uint32_t *thr;       // [S][kClassStride]
     const uint32_t *ztab;      // [S][8] per degree z: u_s thresholds (see run_colored)
     WT *packed;                // [R][2][N]
@@ -110,10 +111,88 @@ __device__ __forceinline__ WT ge_pick(int k, WT c0, WT c1, WT c2)
     }
 }
 
+// "count >= k" for a per-lane k (0..8) without branches: k is a nibble of a
+// packed word, its bits become all-ones / zero masks (signed one-bit field
+// extracts) compared most significant bit first.
+template <typename WT>
+__device__ __forceinline__ WT bitmask(uint32_t pk, int bit)
+{
+    return (WT)(int64_t)((int32_t)(pk << (31 - bit)) >> 31);  // 0 or all ones
+}
+template <typename WT>
+__device__ __forceinline__ WT ge_nib(uint32_t pk, int nib, WT c0, WT c1, WT c2)
+{
+    const int b = 4 * nib;
+    const WT K1 = bitmask<WT>(pk, b + 1), K2 = bitmask<WT>(pk, b + 2);
+    const WT t0 = c0 | ~bitmask<WT>(pk, b);
+    const WT t1 = (c1 & t0) | (~K1 & (c1 | t0));
+    const WT t2 = (c2 & t1) | (~K2 & (c2 | t1));
+    return t2 & ~bitmask<WT>(pk, b + 3);
+}
+
+// Decision classes of one spin with a field h in one field step, for +-1
+// couplings: a bit's class depends only on its spin's sign sigma, its
+// unsatisfied-bond count u_s (0..z) and antiparallel Trotter bonds u_t, via
+//   ed = -2 * fl((sigma*h + (z - 2 u_s)) + J_perp (2 - 2 u_t)),
+// which is exactly the general path's -2 s_i (h_i + sum J s_j + J_perp ts) when
+// the row's partial sums are exact (fields 0 or 2^-27 <= |h| <= 2^20, the
+// problem-level condition).  Classes are non-decreasing in u_s, so each
+// (sigma, u_t) reduces to two thresholds; the first three stochastic
+// (sigma, u_t, u_s) triples keep their draw thresholds.
+struct FieldClasses {
+    uint32_t ka, ks;   // nibble sigma*3 + u_t: accept from u_s >= ka, stochastic from ks
+    int ns;            // stochastic triples
+    uint32_t slot;     // byte t: combo (bits 0-2) and u_s (bits 4-7) of triple t < 3
+    uint32_t st[3];    // their draw thresholds
+};
+
+__device__ __forceinline__ int field_class(double hs, int z, int us, double B, double T, unsigned int *thr)
+{
+    const double X = (hs + (double)(z - 2 * us)) + B;
+    const double ed = -2.0 * X;
+    if (ed > 0.0) return 1;
+    if (T > 0.0 && ed < -23.5 * T) return 0;  // exp(ed/T) < e^-23 < 2^-33: never accepted
+    return piqmc_fixed_class(ed, T, thr);
+}
+
+__device__ __forceinline__ void field_classes(double h, int z, float jp, double T, FieldClasses &fc)
+{
+    fc.ns = 0;
+    fc.ka = fc.ks = fc.slot = 0;
+    fc.st[0] = fc.st[1] = fc.st[2] = 0;
+    for (int cb = 0; cb < 6; ++cb) {
+        const int sg = cb / 3, ut = cb % 3;
+        const double hs = sg ? -h : h;
+        const double B = (double)jp * (double)(2 - 2 * ut);
+        int amin = z + 1, smin, us = z;
+        unsigned int thr = 0;
+        int cls = 0;
+        for (; us >= 0; --us) {
+            cls = field_class(hs, z, us, B, T, &thr);
+            if (cls != 1) break;
+            amin = us;
+        }
+        smin = amin;
+        for (; us >= 0 && cls == 2; --us) {
+            smin = us;
+            if (fc.ns < 3) {
+                fc.slot |= (uint32_t)(cb | (us << 4)) << (8 * fc.ns);
+                if (fc.ns == 0) fc.st[0] = thr;
+                else if (fc.ns == 1) fc.st[1] = thr;
+                else fc.st[2] = thr;
+            }
+            ++fc.ns;
+            if (us > 0) cls = field_class(hs, z, us - 1, B, T, &thr);
+        }
+        fc.ka |= (uint32_t)amin << (4 * cb);
+        fc.ks |= (uint32_t)smin << (4 * cb);
+    }
+}
+
 // Replicas r0 .. r0+nrep-1 of one problem, whole schedule, by one workgroup.
 // UZ: every spin has the same degree a.zuni, so the class thresholds of a
 // field step are wavefront-uniform and the mask selection is scalar code.
-template <typename WT, bool PMJ, bool UZ = false>
+template <typename WT, bool PMJ, bool UZ = false, bool FLD = false>
 __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned char *smem)
 {
     const int N = a.N, W = a.W, tid = threadIdx.x, bd = blockDim.x;
@@ -127,7 +206,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
 
     for (int s = 0; s < a.S; ++s) {
         __syncthreads();
-        if (PMJ) {
+        if (PMJ && !FLD) {
             for (int q = tid; q < kClassStride; q += bd) tthr[q] = a.thr[(size_t)s * kClassStride + q];
             if (tid < 8) tz[tid] = a.ztab[s * 8 + tid];
         }
@@ -157,7 +236,14 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                             for (int z = 0; z < 8; ++z)
                                 if (z < a.zmax) nb[z] = z < zi ? a.cnb[(size_t)z * N + q] : i;
                         }
-                        const uint32_t zt = UZ ? zt_u : (PMJ ? tz[zi] : 0u);
+                        const uint32_t zt = UZ ? zt_u : ((PMJ && !FLD) ? tz[zi] : 0u);
+                        // fields: this spin's class thresholds, shared by the replicas below
+                        double hfield = 0.0;
+                        FieldClasses fc;
+                        if (FLD) {
+                            hfield = (double)a.hq[q];
+                            field_classes(hfield, zi, jp, Ts, fc);
+                        }
                         for (int rr = 0; rr < nrep; ++rr) {
                             WT *mine = st + ((size_t)rr * 2 + tau) * N;
                             const WT *other = st + ((size_t)rr * 2 + 1 - tau) * N;
@@ -182,7 +268,18 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 }
                                 const WT e0 = ~(uL | uR), e1 = uL ^ uR, e2 = uL & uR;
                                 WT X;
-                                if (UZ) {
+                                if (FLD) {
+                                    // per-spin thresholds for (sigma, u_t): sigma = +1 bits are ~S_
+                                    flips = 0;
+                                    X = 0;
+#pragma unroll
+                                    for (int cb = 0; cb < 6; ++cb) {
+                                        const WT eu = (cb % 3) == 0 ? e0 : ((cb % 3) == 1 ? e1 : e2);
+                                        const WT m = (cb < 3 ? ~S_ : S_) & eu;
+                                        flips |= m & ge_nib<WT>(fc.ka, cb, c0, c1, c2);
+                                        X |= m & ge_nib<WT>(fc.ks, cb, c0, c1, c2);
+                                    }
+                                } else if (UZ) {
                                     // thresholds are wave-uniform: every "count >= k" mask
                                     // once, then six selections by a uniform index
                                     WT ge[9];
@@ -215,24 +312,67 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                     WT lt = 0, und = X;
                                     // The stochastic classes of this word (one per field step on
                                     // a regular lattice, rarely more): masks and thresholds in
-                                    // registers; a fourth class falls back to the table walk.
+                                    // registers; more classes take the slow paths below.
                                     WT cm0 = 0, cm1 = 0, cm2 = 0;
                                     uint32_t ct0 = 0, ct1 = 0, ct2 = 0;
                                     int ns = 0;
+                                    if (FLD) {
+                                        ns = fc.ns;
 #pragma unroll
-                                    for (int ut = 0; ut < 3; ++ut) {
-                                        const WT eu = ut == 0 ? e0 : (ut == 1 ? e1 : e2);
-                                        const int amin = (int)((zt >> (8 * ut)) & 15);
-                                        for (int us = (int)((zt >> (8 * ut + 4)) & 15); us < amin; ++us) {
-                                            const WT mk = eu & ge_pick(us, c0, c1, c2) & ~ge_pick(us + 1, c0, c1, c2);
-                                            const uint32_t tk = tthr[(zi * 8 + us) * 3 + ut];
-                                            if (ns == 0) { cm0 = mk; ct0 = tk; }
-                                            else if (ns == 1) { cm1 = mk; ct1 = tk; }
-                                            else if (ns == 2) { cm2 = mk; ct2 = tk; }
-                                            ++ns;
+                                        for (int t = 0; t < 3; ++t) {
+                                            if (t < ns) {
+                                                const uint32_t sb = (fc.slot >> (8 * t)) & 255u;
+                                                const int cb = (int)(sb & 7u), us = (int)(sb >> 4);
+                                                const WT eu = (cb % 3) == 0 ? e0 : ((cb % 3) == 1 ? e1 : e2);
+                                                const uint32_t kk = (uint32_t)us | ((uint32_t)(us + 1) << 4);
+                                                const WT mk = (cb < 3 ? ~S_ : S_) & eu & ge_nib<WT>(kk, 0, c0, c1, c2) &
+                                                              ~ge_nib<WT>(kk, 1, c0, c1, c2);
+                                                if (t == 0) { cm0 = mk; ct0 = fc.st[0]; }
+                                                else if (t == 1) { cm1 = mk; ct1 = fc.st[1]; }
+                                                else { cm2 = mk; ct2 = fc.st[2]; }
+                                            }
+                                        }
+                                    } else {
+#pragma unroll
+                                        for (int ut = 0; ut < 3; ++ut) {
+                                            const WT eu = ut == 0 ? e0 : (ut == 1 ? e1 : e2);
+                                            const int amin = (int)((zt >> (8 * ut)) & 15);
+                                            for (int us = (int)((zt >> (8 * ut + 4)) & 15); us < amin; ++us) {
+                                                const WT mk = eu & ge_pick(us, c0, c1, c2) & ~ge_pick(us + 1, c0, c1, c2);
+                                                const uint32_t tk = tthr[(zi * 8 + us) * 3 + ut];
+                                                if (ns == 0) { cm0 = mk; ct0 = tk; }
+                                                else if (ns == 1) { cm1 = mk; ct1 = tk; }
+                                                else if (ns == 2) { cm2 = mk; ct2 = tk; }
+                                                ++ns;
+                                            }
                                         }
                                     }
                                     constexpr int kPer = sizeof(WT) == 4 ? 4 : 2;  // words per Philox call
+                                    if (FLD && ns > 3) {
+                                        // many stochastic triples (high temperature): each
+                                        // stochastic bit assembles its own uniform and threshold
+                                        for (int mm = 0; mm < W; ++mm) {
+                                            if (!((X >> mm) & WT(1))) continue;
+                                            const int sg = (int)((S_ >> mm) & WT(1));
+                                            const int ut = (int)((uL >> mm) & WT(1)) + (int)((uR >> mm) & WT(1));
+                                            const int us = (int)((c0 >> mm) & WT(1)) + 2 * (int)((c1 >> mm) & WT(1)) +
+                                                           4 * (int)((c2 >> mm) & WT(1));
+                                            unsigned int thr = 0;
+                                            (void)field_class(sg ? -hfield : hfield, zi, us,
+                                                              (double)jp * (double)(2 - 2 * ut), Ts, &thr);
+                                            uint32_t u = 0;
+                                            for (int qd = 0; qd < 32 / kPer; ++qd) {
+                                                uint32_t out[4];
+                                                philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd,
+                                                              a.key0, a.key1, out);
+#pragma unroll
+                                                for (int t = 0; t < kPer; ++t)
+                                                    u |= (uint32_t)((rword<WT>(out, t) >> mm) & WT(1)) << (31 - (qd * kPer + t));
+                                            }
+                                            if (u < thr) lt |= WT(1) << mm;
+                                        }
+                                        und = 0;
+                                    }
                                     for (int qd = 0; qd < 32 / kPer && und; ++qd) {
                                         uint32_t out[4];
                                         philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd,
@@ -340,12 +480,12 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
     for (int q = tid; q < 2 * N * nrep; q += bd) gst[q] = st[q];
 }
 
-template <typename WT, bool PMJ, bool UZ>
+template <typename WT, bool PMJ, bool UZ, bool FLD = false>
 __global__ void colored_kernel(ColArgs<WT> a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int r0 = blockIdx.x * a.rpw;
-    colored_body<WT, PMJ, UZ>(a, r0, min(a.rpw, a.R - r0), smem);
+    colored_body<WT, PMJ, UZ, FLD>(a, r0, min(a.rpw, a.R - r0), smem);
 }
 
 // ------------------------------------------------------------------ dense path
@@ -619,11 +759,15 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
     a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
     a.replica_base = rbase; a.sweep_base = sbase;
     a.jperp = d_jp; a.rows = pb->d_rows; a.order = pb->d_order; a.color_off = d_coff;
-    a.cnb = pb->d_cnb; a.cinfo = pb->d_cinfo; a.thr = d_thr; a.ztab = d_zt;
+    a.cnb = pb->d_cnb; a.cinfo = pb->d_cinfo; a.thr = d_thr; a.ztab = d_zt; a.hq = pb->d_hq;
     a.packed = d_packed; a.etrace = etrace;
     const int grid = (R + rpw - 1) / rpw;
     piqmc::timing_begin(st);
-    if (pmj) {
+    if (!pmj && pb->pmjf) {
+        a.zuni = -1;
+        (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        colored_kernel<WT, true, false, true><<<grid, block, lds, st>>>(a);
+    } else if (pmj) {
         if (a.zuni >= 0) {
             (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             colored_kernel<WT, true, true><<<grid, block, lds, st>>>(a);

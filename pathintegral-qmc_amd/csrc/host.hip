@@ -255,24 +255,34 @@ static int upload_coloring(piqmc_problem &p)
     if (p.d_order) (void)hipFree(p.d_order);
     if (p.d_cnb) (void)hipFree(p.d_cnb);
     if (p.d_cinfo) (void)hipFree(p.d_cinfo);
-    p.d_order = nullptr; p.d_cnb = nullptr; p.d_cinfo = nullptr;
+    if (p.d_hq) (void)hipFree(p.d_hq);
+    p.d_order = nullptr; p.d_cnb = nullptr; p.d_cinfo = nullptr; p.d_hq = nullptr;
     PIQMC_HIP(hipMalloc(&p.d_order, sizeof(int32_t) * N));
     PIQMC_HIP(hipMemcpy(p.d_order, order.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
-    if (p.pmj) {
+    if (p.pmj || p.pmjf) {
         const int zm = std::max(p.zmax, 1);
         std::vector<int32_t> cnb((size_t)zm * N, 0);
         std::vector<uint32_t> cinfo(N, 0);
+        std::vector<float> hq(N, 0.0f);
         for (int q = 0; q < N; ++q) {
             int i = order[q], z = 0;
             uint32_t neg = 0;
             for (int e = 0; e < p.Z; ++e) {
                 float Jv = p.J[(size_t)i * p.Z + e];
                 if (Jv == 0.0f) continue;
+                if (p.idx[(size_t)i * p.Z + e] == i) {  // the field (only with pmjf)
+                    hq[q] = Jv;
+                    continue;
+                }
                 cnb[(size_t)z * N + q] = p.idx[(size_t)i * p.Z + e];
                 if (Jv < 0.0f) neg |= 1u << z;
                 ++z;
             }
             cinfo[q] = neg | ((uint32_t)z << 8);
+        }
+        if (p.pmjf) {
+            PIQMC_HIP(hipMalloc(&p.d_hq, sizeof(float) * N));
+            PIQMC_HIP(hipMemcpy(p.d_hq, hq.data(), sizeof(float) * N, hipMemcpyHostToDevice));
         }
         PIQMC_HIP(hipMalloc(&p.d_cnb, sizeof(int32_t) * cnb.size()));
         PIQMC_HIP(hipMemcpy(p.d_cnb, cnb.data(), sizeof(int32_t) * cnb.size(), hipMemcpyHostToDevice));
@@ -293,9 +303,11 @@ extern "C" int piqmc_problem_create(int N, int Z, const double *nbs, piqmc_probl
     p->idx.resize((size_t)N * Z);
     p->J.resize((size_t)N * Z);
     p->pmj = 1;
+    p->pmjf = 1;
+    int nfield = 0;
     p->zmax = 0;
     for (int i = 0; i < N; ++i) {
-        int z = 0;
+        int z = 0, fields = 0;
         for (int e = 0; e < Z; ++e) {
             double jd = nbs[((size_t)i * Z + e) * 2];
             if (!(jd > -1.0) || !(jd < (double)N)) {  // (int) truncation must land in [0, N)
@@ -311,13 +323,24 @@ extern "C" int piqmc_problem_create(int N, int Z, const double *nbs, piqmc_probl
             p->idx[(size_t)i * Z + e] = j;
             p->J[(size_t)i * Z + e] = Jv;
             if (Jv != 0.0f) {
-                if (j == i || (Jv != 1.0f && Jv != -1.0f)) p->pmj = 0;
-                else ++z;
+                if (j == i) {
+                    p->pmj = 0;
+                    ++fields;
+                    ++nfield;
+                    const float ah = fabsf(Jv);
+                    if (fields > 1 || ah < 0x1p-27f || ah > 0x1p20f) p->pmjf = 0;
+                } else if (Jv != 1.0f && Jv != -1.0f) {
+                    p->pmj = 0;
+                    p->pmjf = 0;
+                } else {
+                    ++z;
+                }
             }
         }
         p->zmax = std::max(p->zmax, z);
     }
-    if (p->zmax > 7) p->pmj = 0;  // bit-sliced counter holds 0..7
+    if (p->zmax > 7) p->pmj = p->pmjf = 0;  // bit-sliced counter holds 0..7
+    if (p->pmj || nfield == 0) p->pmjf = 0;
     p->zuni = p->zmax;
     for (int i = 0; i < N && p->zuni >= 0; ++i) {
         int z = 0;
@@ -401,6 +424,7 @@ extern "C" int piqmc_problem_destroy(piqmc_problem *p)
     if (p->d_order) (void)hipFree(p->d_order);
     if (p->d_cnb) (void)hipFree(p->d_cnb);
     if (p->d_cinfo) (void)hipFree(p->d_cinfo);
+    if (p->d_hq) (void)hipFree(p->d_hq);
     delete p;
     return PIQMC_OK;
 }
@@ -411,7 +435,7 @@ extern "C" int piqmc_problem_info(const piqmc_problem *p, int *n, int *z, int *n
     if (n) *n = p->N;
     if (z) *z = p->Z;
     if (nc) *nc = p->ncolors;
-    if (pmj) *pmj = p->pmj;
+    if (pmj) *pmj = p->pmj ? 1 : (p->pmjf ? 2 : 0);
     if (zmax) *zmax = p->zmax;
     return PIQMC_OK;
 }

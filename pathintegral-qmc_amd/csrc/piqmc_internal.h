@@ -51,6 +51,10 @@ struct piqmc_problem {
     int sweep_path = 0;           // parallel sweep: 0 auto, 1 word per thread, 2 dense (lane per slice)
     int32_t *d_cnb = nullptr;     // [zmax][N] neighbour of position q (color order)
     uint32_t *d_cinfo = nullptr;  // [N] bits 0..7 negative-coupling mask, 8..11 z
+    // +-1 couplings with fields: as above, plus at most one field per spin with
+    // 2^-27 <= |h| <= 2^20 (so the local field's partial sums are exact)
+    int pmjf = 0;
+    float *d_hq = nullptr;        // [N] field of spin order[q]
 };
 
 // ------------------------------------------------------------------ device helpers

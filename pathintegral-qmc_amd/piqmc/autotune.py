@@ -50,8 +50,9 @@ def _arch():
 
 
 def key(pb, P, R):
+    path = 1 if pb.pm_j else (2 if pb.pm_j_fields else 0)
     return "%s|N%d|Z%d|zmax%d|pmj%d|nc%d|P%d|R%d" % (_arch(), pb.nspins, pb.maxnb, pb.zmax,
-                                                     int(pb.pm_j), pb.ncolors, P, R)
+                                                     path, pb.ncolors, P, R)
 
 
 def candidates(pb, P, R):

@@ -103,7 +103,9 @@ class Problem:
         vals = [C.c_int() for _ in range(5)]
         _lib.call("piqmc_problem_info", self._h, *[C.byref(v) for v in vals])
         self.maxnb = vals[1].value
-        self.ncolors, self.pm_j, self.zmax = vals[2].value, bool(vals[3].value), vals[4].value
+        self.ncolors, self.zmax = vals[2].value, vals[4].value
+        self.pm_j = vals[3].value == 1            # +-1 couplings only
+        self.pm_j_fields = vals[3].value == 2     # +-1 couplings and fields
 
     def close(self):
         if getattr(self, "_h", None):

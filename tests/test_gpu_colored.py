@@ -57,11 +57,29 @@ def test_colored_3d_pm_j(gpu):
     assert np.array_equal(got, ref) and np.array_equal(etg, etr)
 
 
+@pytest.mark.parametrize("P,T,S", [(16, 0.05, 15), (128, 1 / 128, 12), (8, 2.0, 10), (32, 0.5, 10)])
+def test_colored_fields_path(gpu, P, T, S):
+    # 3-D +-J with Gaussian fields (the large sparse workload's structure): the
+    # bit-sliced path with per-spin thresholds; T=2 and 0.5 give many
+    # stochastic classes per spin (the per-bit fallback)
+    N, nbs, _, _, _ = pm_j_problem(4, dim=3, seed=4 + P, field_sigma=0.1)
+    pb, got, ref, etg, etr = _run(gpu, nbs, P, np.linspace(3.0, 1e-8, S), 1, T, R=3, seed=8 + P,
+                                  rbase=2, sbase=5)
+    assert pb.pm_j_fields and not pb.pm_j
+    assert np.array_equal(got, ref)
+    np.testing.assert_allclose(etg, etr, rtol=1e-12, atol=1e-12)
+
+
 def test_colored_general_path_fields(gpu):
-    # 3-D +-J with Gaussian fields (the large sparse workload's structure)
+    # a field below 2^-27 (inexact partial sums) keeps the problem on the
+    # per-bit general path, which must agree as well
     N, nbs, _, _, _ = pm_j_problem(4, dim=3, seed=4, field_sigma=0.1)
+    nbs = nbs.copy()
+    for e in range(nbs.shape[1]):
+        if int(nbs[5, e, 0]) == 5 and nbs[5, e, 1] != 0:
+            nbs[5, e, 1] = 1e-9
     pb, got, ref, etg, etr = _run(gpu, nbs, 16, np.linspace(3.0, 1e-8, 15), 1, 0.05, R=3, seed=8)
-    assert not pb.pm_j
+    assert not pb.pm_j and not pb.pm_j_fields
     assert np.array_equal(got, ref)
     np.testing.assert_allclose(etg, etr, rtol=1e-12, atol=1e-12)
 
