@@ -146,6 +146,38 @@ def c5_leg(steps=2000, replicas=1024):
                         "%d replicas, Gamma 3.0->1e-8 over %d steps (parallel sweep)" % (replicas, steps)}
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
+
+
+def gemm_leg(shapes=((1024, 1024, 1728), (1024, 1024, 1024), (1024, 1728, 1024),
+                     (2048, 512, 256), (2048, 512, 512), (2048, 256, 512)), reps=20):
+    """The network's BF16 MFMA GEMM (C = A B^T, FP32 accumulation) at the
+    shapes of the two MADE workloads (batch x out x in), timed with HIP events
+    on the stream the kernel runs on."""
+    import ctypes as C
+    import torch
+    from piqmc import _lib
+    out = []
+    st = torch.cuda.current_stream()
+    for M, N, K in shapes:
+        A = torch.randint(0, 1 << 15, (M, K), dtype=torch.int32, device="cuda").to(torch.int16)
+        B = torch.randint(0, 1 << 15, (N, K), dtype=torch.int32, device="cuda").to(torch.int16)
+        Cm = torch.empty((M, N), dtype=torch.float32, device="cuda")
+        args = (A.data_ptr(), B.data_ptr(), M, N, K, Cm.data_ptr(), C.c_void_p(st.cuda_stream))
+        _lib.call("piqmc_gemm_bf16_nt", *args)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            _lib.call("piqmc_gemm_bf16_nt", *args)
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        tf = 2.0 * M * N * K / (ms * 1e-3) / 1e12
+        out.append({"M": M, "N": N, "K": K, "us": ms * 1e3, "tflops": tf,
+                    "mfma_frac": tf / MFMA_BF16_PEAK_TFLOPS})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,12 +306,18 @@ def main():
         out["batch"] = batch_leg()
     if args.c5:
         out["c5"] = c5_leg()
+        if not args.no_learned:
+            from piqmc import learned
+            out["c5_learned"] = learned.run(L=12, dim=3, field_sigma=0.1, P=128, R=1024, hidden=1024,
+                                            train_steps=400, epochs=2, eval_steps=(25, 50, 100),
+                                            every=4)
     if not args.no_learned:
+        out["mfma_gemm"] = gemm_leg()
         from piqmc import learned
         lr = learned.run(eval_steps=(25, 50, 100, 200))
         out["learned"] = {k: lr[k] for k in ("workload", "train_slices", "train_nll_per_epoch",
-                                             "train_seconds", "e_best_per_spin", "curves",
-                                             "time_ratio")}
+                                             "train_seconds", "train_tflops", "e_best_per_spin",
+                                             "curves", "time_ratio")}
 
     if not args.no_cpu:
         single, omp, nthr, sample = cpu_baseline(nbs, N, P, np.float32(T), sched, reps=2)

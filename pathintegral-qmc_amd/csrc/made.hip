@@ -98,8 +98,8 @@ __global__ __launch_bounds__(256) void gemm_nt_bf16(const uint16_t *A, int lda, 
         for (int t = 0; t < 2; ++t) {
             uint4 va = *reinterpret_cast<const uint4 *>(&As[wm + t * 16 + (lane & 15)][(lane >> 4) * 8]);
             uint4 vb = *reinterpret_cast<const uint4 *>(&Bs[wn + t * 16 + (lane & 15)][(lane >> 4) * 8]);
-            af[t] = *reinterpret_cast<bf16x8 *>(&va);
-            bfr[t] = *reinterpret_cast<bf16x8 *>(&vb);
+            af[t] = __builtin_bit_cast(bf16x8, va);
+            bfr[t] = __builtin_bit_cast(bf16x8, vb);
         }
 #pragma unroll
         for (int a = 0; a < 2; ++a)
@@ -128,11 +128,102 @@ __global__ __launch_bounds__(256) void gemm_nt_bf16(const uint16_t *A, int lda, 
         }
 }
 
+// Pipelined form for K, lda, ldb multiples of 64 / 8 / 8: K-stages of 64
+// (two MFMA k-steps), LDS double-buffered, the next stage's global loads in
+// flight while the current stage's MFMAs run, one barrier per stage.  Each
+// thread moves 2 x 16 B of A and of B per stage; rows past M / N load zeros.
+constexpr int PK = 64, PKP = PK + 8;  // stage depth; LDS row pitch (bf16)
+
+__global__ __launch_bounds__(256) void gemm_nt_bf16_pipe(const uint16_t *A, int lda, const uint16_t *B, int ldb,
+                                                         int M, int N, int K, const float *bias, int relu,
+                                                         float *Cf, uint16_t *Cb, int ldc)
+{
+    __shared__ __align__(16) uint16_t As[2][GT][PKP];
+    __shared__ __align__(16) uint16_t Bs[2][GT][PKP];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // staging map: thread -> rows r and r+32, 8 consecutive k at column c
+    const int sr = tid >> 3, sc = (tid & 7) * 8;
+    const bool a0 = m0 + sr < M, a1 = m0 + sr + 32 < M, b0 = n0 + sr < N, b1 = n0 + sr + 32 < N;
+    const uint16_t *pa0 = A + (size_t)(a0 ? m0 + sr : 0) * lda + sc;
+    const uint16_t *pa1 = A + (size_t)(a1 ? m0 + sr + 32 : 0) * lda + sc;
+    const uint16_t *pb0 = B + (size_t)(b0 ? n0 + sr : 0) * ldb + sc;
+    const uint16_t *pb1 = B + (size_t)(b1 ? n0 + sr + 32 : 0) * ldb + sc;
+    uint4 ra0, ra1, rb0, rb1;
+    auto fetch = [&](int k0) {  // rows past M / N read row 0 (in bounds); zeroed when stored
+        ra0 = *reinterpret_cast<const uint4 *>(pa0 + k0);
+        ra1 = *reinterpret_cast<const uint4 *>(pa1 + k0);
+        rb0 = *reinterpret_cast<const uint4 *>(pb0 + k0);
+        rb1 = *reinterpret_cast<const uint4 *>(pb1 + k0);
+    };
+    const uint32_t ma0 = a0 ? ~0u : 0u, ma1 = a1 ? ~0u : 0u, mb0 = b0 ? ~0u : 0u, mb1 = b1 ? ~0u : 0u;
+    auto masked = [](uint4 v, uint32_t m) { return make_uint4(v.x & m, v.y & m, v.z & m, v.w & m); };
+    auto stash = [&](int buf) {
+        *reinterpret_cast<uint4 *>(&As[buf][sr][sc]) = masked(ra0, ma0);
+        *reinterpret_cast<uint4 *>(&As[buf][sr + 32][sc]) = masked(ra1, ma1);
+        *reinterpret_cast<uint4 *>(&Bs[buf][sr][sc]) = masked(rb0, mb0);
+        *reinterpret_cast<uint4 *>(&Bs[buf][sr + 32][sc]) = masked(rb1, mb1);
+    };
+    const int ns = K / PK;
+    fetch(0);
+    stash(0);
+    __syncthreads();
+    for (int s = 0; s < ns; ++s) {
+        const int buf = s & 1;
+        if (s + 1 < ns) fetch((s + 1) * PK);
+#pragma unroll
+        for (int kk = 0; kk < PK; kk += 32) {
+            bf16x8 af[2], bfr[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                uint4 va = *reinterpret_cast<const uint4 *>(&As[buf][wm + t * 16 + (lane & 15)][kk + (lane >> 4) * 8]);
+                uint4 vb = *reinterpret_cast<const uint4 *>(&Bs[buf][wn + t * 16 + (lane & 15)][kk + (lane >> 4) * 8]);
+                af[t] = __builtin_bit_cast(bf16x8, va);
+                bfr[t] = __builtin_bit_cast(bf16x8, vb);
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        }
+        if (s + 1 < ns) stash(buf ^ 1);
+        __syncthreads();
+    }
+    // accumulator layout: lane holds column (lane&15), rows (lane>>4)*4 + j
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int col = n0 + wn + b * 16 + (lane & 15);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = m0 + wm + a * 16 + (lane >> 4) * 4 + j;
+                if (row < M && col < N) {
+                    float v = acc[a][b][j];
+                    if (bias) v += bias[col];
+                    if (relu) v = v > 0.f ? v : 0.f;
+                    if (Cf) Cf[(size_t)row * ldc + col] = v;
+                    if (Cb) Cb[(size_t)row * ldc + col] = f2bf_bits(v);
+                }
+            }
+        }
+}
+
 void gemm(const uint16_t *A, int lda, const uint16_t *B, int ldb, int M, int N, int K, const float *bias,
           int relu, float *Cf, uint16_t *Cb, int ldc, hipStream_t st)
 {
     dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT);
-    gemm_nt_bf16<<<grid, 256, 0, st>>>(A, lda, B, ldb, M, N, K, bias, relu, Cf, Cb, ldc);
+    if (K % PK == 0 && lda % 8 == 0 && ldb % 8 == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0)
+        gemm_nt_bf16_pipe<<<grid, 256, 0, st>>>(A, lda, B, ldb, M, N, K, bias, relu, Cf, Cb, ldc);
+    else
+        gemm_nt_bf16<<<grid, 256, 0, st>>>(A, lda, B, ldb, M, N, K, bias, relu, Cf, Cb, ldc);
 }
 
 // ------------------------------------------------------------------ elementwise

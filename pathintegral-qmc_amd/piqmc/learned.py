@@ -1,11 +1,13 @@
 """Plain vs learned-proposal annealing at equal GPU wall time.
 
-Workload: 2-D +-J glass L=16 (N=256), P=32, T=1/32 (P*T=1), 2048 replicas,
+Default workload: 2-D +-J glass L=16 (N=256), P=32, T=1/32 (P*T=1), 2048 replicas,
 Gamma linear 3.0 -> 1e-8.  A MADE 256->512->512->256 (BF16, FP32 accumulation,
 Adam lr 1e-3) is trained on ~1e5 slices that plain anneals emit at Gamma <= 0.5,
 then used as an independence MH slice proposal every 4 sweeps.  Residual
 energy per spin = mean over replicas of (min over slices E - E_best) / N,
-with E_best the lowest energy any run reached.
+with E_best the lowest energy any run reached.  ``dim`` and ``field_sigma``
+select other lattices (the 3-D L=12 field workload uses dim=3, field_sigma=0.1,
+P=128, 1024 replicas and a 1728->1024->1024->1728 network).
 """
 import time
 
@@ -22,12 +24,13 @@ def _timed(fn):
 
 
 def run(L=16, P=32, R=2048, hidden=512, train_steps=500, epochs=5, lr=1e-3,
-        eval_steps=(25, 50, 100, 200, 400), every=4, seed=0, device="cuda"):
+        eval_steps=(25, 50, 100, 200, 400), every=4, seed=0, device="cuda", dim=2,
+        field_sigma=0.0):
     import torch
     from . import device as D, instances
     from .made import MADE, anneal_with_proposals
 
-    N, r, c, v = instances.lattice_pm_j(L, 2, np.random.RandomState(seed + 1))
+    N, r, c, v = instances.lattice_pm_j(L, dim, np.random.RandomState(seed + 1), field_sigma)
     pb = D.Problem(instances.neighbors(N, r, c, v))
     T = 1.0 / P
     g = torch.Generator(device="cpu").manual_seed(seed)
@@ -76,9 +79,12 @@ def run(L=16, P=32, R=2048, hidden=512, train_steps=500, epochs=5, lr=1e-3,
     target = min(p["residual_per_spin"] for p in res["plain"])
     t_plain = min(p["seconds"] for p in res["plain"] if p["residual_per_spin"] <= target)
     hits = [q["seconds"] for q in res["learned"] if q["residual_per_spin"] <= target]
-    out = {"workload": "2-D +-J L=%d N=%d, P=%d, T=1/%d, %d replicas, MADE %d->%d->%d->%d" % (
-        L, N, P, P, R, N, hidden, hidden, N),
+    out = {"workload": "%d-D +-J L=%d N=%d%s, P=%d, T=1/%d, %d replicas, MADE %d->%d->%d->%d" % (
+        dim, L, N, ", h~N(0,%g^2)" % field_sigma if field_sigma else "", P, P, R, N, hidden, hidden, N),
         "train_slices": int(data.shape[0]), "train_nll_per_epoch": hist, "train_seconds": t_train,
+        # forward 2(N H1 + H1 H2 + H2 N) per slice, backward about twice that
+        "train_tflops": 3 * 2.0 * (N * hidden + hidden * hidden + hidden * N) * int(data.shape[0]) * epochs
+                        / t_train / 1e12,
         "e_best_per_spin": float(best / N), "curves": res,
         "plain_time_to_best": t_plain,
         "learned_time_to_plain_best": min(hits) if hits else None,
