@@ -192,7 +192,9 @@ __device__ __forceinline__ void field_classes(double h, int z, float jp, double 
 // Replicas r0 .. r0+nrep-1 of one problem, whole schedule, by one workgroup.
 // UZ: every spin has the same degree a.zuni, so the class thresholds of a
 // field step are wavefront-uniform and the mask selection is scalar code.
-template <typename WT, bool PMJ, bool UZ = false, bool FLD = false>
+// ZU > 0: the uniform degree is known at compile time (4 for square, 6 for
+// cubic lattices), so the neighbour loops have fixed trip counts.
+template <typename WT, bool PMJ, bool UZ = false, bool FLD = false, int ZU = 0>
 __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned char *smem)
 {
     const int N = a.N, W = a.W, tid = threadIdx.x, bd = blockDim.x;
@@ -231,10 +233,15 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                         int nb[8];
                         if (PMJ) {
                             info = a.cinfo[q];
-                            zi = UZ ? a.zuni : (int)((info >> 8) & 15);
+                            zi = ZU > 0 ? ZU : (UZ ? a.zuni : (int)((info >> 8) & 15));
+                            if (ZU > 0) {
 #pragma unroll
-                            for (int z = 0; z < 8; ++z)
-                                if (z < a.zmax) nb[z] = z < zi ? a.cnb[(size_t)z * N + q] : i;
+                                for (int z = 0; z < ZU; ++z) nb[z] = a.cnb[(size_t)z * N + q];
+                            } else {
+#pragma unroll
+                                for (int z = 0; z < 8; ++z)
+                                    if (z < a.zmax) nb[z] = z < zi ? a.cnb[(size_t)z * N + q] : i;
+                            }
                         }
                         const uint32_t zt = UZ ? zt_u : ((PMJ && !FLD) ? tz[zi] : 0u);
                         // fields: this spin's class thresholds, shared by the replicas below
@@ -257,7 +264,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 WT c0 = 0, c1 = 0, c2 = 0;  // bit-sliced unsatisfied-bond counter
 #pragma unroll
                                 for (int z = 0; z < 8; ++z) {
-                                    if (z < a.zmax && z < zi) {
+                                    if (ZU > 0 ? z < ZU : (z < a.zmax && z < zi)) {
                                         const WT x = S_ ^ mine[nb[z]] ^ (((info >> z) & 1u) ? ~WT(0) : WT(0));
                                         const WT k0 = c0 & x;
                                         c0 ^= x;
@@ -480,12 +487,12 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
     for (int q = tid; q < 2 * N * nrep; q += bd) gst[q] = st[q];
 }
 
-template <typename WT, bool PMJ, bool UZ, bool FLD = false>
+template <typename WT, bool PMJ, bool UZ, bool FLD = false, int ZU = 0>
 __global__ void colored_kernel(ColArgs<WT> a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int r0 = blockIdx.x * a.rpw;
-    colored_body<WT, PMJ, UZ, FLD>(a, r0, min(a.rpw, a.R - r0), smem);
+    colored_body<WT, PMJ, UZ, FLD, ZU>(a, r0, min(a.rpw, a.R - r0), smem);
 }
 
 // ------------------------------------------------------------------ dense path
@@ -768,7 +775,13 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         colored_kernel<WT, true, false, true><<<grid, block, lds, st>>>(a);
     } else if (pmj) {
-        if (a.zuni >= 0) {
+        if (a.zuni == 4) {
+            (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true, false, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            colored_kernel<WT, true, true, false, 4><<<grid, block, lds, st>>>(a);
+        } else if (a.zuni == 6) {
+            (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true, false, 6>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            colored_kernel<WT, true, true, false, 6><<<grid, block, lds, st>>>(a);
+        } else if (a.zuni >= 0) {
             (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             colored_kernel<WT, true, true><<<grid, block, lds, st>>>(a);
         } else {
