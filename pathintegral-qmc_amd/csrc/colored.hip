@@ -23,6 +23,8 @@
 // for bit m, so the result does not depend on workgroup size or scheduling.
 #include <math.h>
 
+#include <algorithm>
+
 #include "piqmc_internal.h"
 
 namespace {
@@ -194,7 +196,10 @@ __device__ __forceinline__ void field_classes(double h, int z, float jp, double 
 // field step are wavefront-uniform and the mask selection is scalar code.
 // ZU > 0: the uniform degree is known at compile time (4 for square, 6 for
 // cubic lattices), so the neighbour loops have fixed trip counts.
-template <typename WT, bool PMJ, bool UZ = false, bool FLD = false, int ZU = 0>
+// CACHED (with ZU > 0): two colors of at most two spins per thread each; every
+// thread keeps its spins' indices and neighbour lists in registers for the
+// whole schedule instead of reloading them in every phase.
+template <typename WT, bool PMJ, bool UZ = false, bool FLD = false, int ZU = 0, bool CACHED = false>
 __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned char *smem)
 {
     const int N = a.N, W = a.W, tid = threadIdx.x, bd = blockDim.x;
@@ -205,6 +210,24 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
     const WT valid = (W == (int)(8 * sizeof(WT))) ? ~WT(0) : ((WT(1) << W) - 1);
     WT *gst = a.packed + (size_t)r0 * 2 * N;
     for (int q = tid; q < 2 * N * nrep; q += bd) st[q] = gst[q];
+    constexpr int ZC = ZU > 0 ? ZU : 1;
+    int cq[2][2], cI[2][2], cN[2][2][ZC];
+    uint32_t cF[2][2];
+    if (CACHED) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int q = a.color_off[c] + tid + k * bd;
+                const bool in = q < a.color_off[c + 1];
+                const int qs = in ? q : 0;
+                cq[c][k] = in ? q : -1;
+                cI[c][k] = a.order[qs];
+                cF[c][k] = a.cinfo[qs];
+#pragma unroll
+                for (int z = 0; z < ZC; ++z) cN[c][k][z] = a.cnb[(size_t)z * N + qs];
+            }
+    }
 
     for (int s = 0; s < a.S; ++s) {
         __syncthreads();
@@ -219,30 +242,34 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
         const int ka0 = min((int)(zt_u & 15), 8), ks0 = min((int)((zt_u >> 4) & 15), 8);
         const int ka1 = min((int)((zt_u >> 8) & 15), 8), ks1 = min((int)((zt_u >> 12) & 15), 8);
         const int ka2 = min((int)((zt_u >> 16) & 15), 8), ks2 = min((int)((zt_u >> 20) & 15), 8);
+        // and its stochastic (u_t, u_s) classes with their draw thresholds (the
+        // first three; more take the table walk)
+        int un = 0, ut_s[3] = {0, 0, 0}, us_s[3] = {0, 0, 0};
+        uint32_t th_s[3] = {0u, 0u, 0u};
+        if (PMJ && UZ && !FLD) {
+#pragma unroll
+            for (int ut = 0; ut < 3; ++ut) {
+                const int lo = ut == 0 ? ks0 : (ut == 1 ? ks1 : ks2), hi = ut == 0 ? ka0 : (ut == 1 ? ka1 : ka2);
+                for (int us = lo; us < hi; ++us) {
+                    if (un < 3) {
+                        const uint32_t th = a.thr[(size_t)s * kClassStride + (a.zuni * 8 + us) * 3 + ut];
+                        if (un == 0) { ut_s[0] = ut; us_s[0] = us; th_s[0] = th; }
+                        else if (un == 1) { ut_s[1] = ut; us_s[1] = us; th_s[1] = th; }
+                        else { ut_s[2] = ut; us_s[2] = us; th_s[2] = th; }
+                    }
+                    ++un;
+                }
+            }
+        }
         const float jp = a.jperp[s];
         const double Ts = (double)a.temps[s];
         for (int m = 0; m < a.M; ++m) {
             const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
             for (int tau = 0; tau < 2; ++tau) {
-                for (int c = 0; c < a.ncolors; ++c) {
-                    const int q0 = a.color_off[c], q1 = a.color_off[c + 1];
-                    for (int q = q0 + tid; q < q1; q += bd) {
-                        const int i = a.order[q];
-                        uint32_t info = 0;
-                        int zi = 0;
-                        int nb[8];
-                        if (PMJ) {
-                            info = a.cinfo[q];
-                            zi = ZU > 0 ? ZU : (UZ ? a.zuni : (int)((info >> 8) & 15));
-                            if (ZU > 0) {
-#pragma unroll
-                                for (int z = 0; z < ZU; ++z) nb[z] = a.cnb[(size_t)z * N + q];
-                            } else {
-#pragma unroll
-                                for (int z = 0; z < 8; ++z)
-                                    if (z < a.zmax) nb[z] = z < zi ? a.cnb[(size_t)z * N + q] : i;
-                            }
-                        }
+                // one spin (position q in color order, index i, neighbours nb)
+                // for every replica of the workgroup
+                auto spin = [&](const int q, const int i, const uint32_t info, const int zi,
+                                const int *nb) __attribute__((always_inline)) {
                         const uint32_t zt = UZ ? zt_u : ((PMJ && !FLD) ? tz[zi] : 0u);
                         // fields: this spin's class thresholds, shared by the replicas below
                         double hfield = 0.0;
@@ -275,6 +302,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 }
                                 const WT e0 = ~(uL | uR), e1 = uL ^ uR, e2 = uL & uR;
                                 WT X;
+                                WT ge[9];  // "count >= k" masks (uniform-degree path)
                                 if (FLD) {
                                     // per-spin thresholds for (sigma, u_t): sigma = +1 bits are ~S_
                                     flips = 0;
@@ -289,7 +317,6 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 } else if (UZ) {
                                     // thresholds are wave-uniform: every "count >= k" mask
                                     // once, then six selections by a uniform index
-                                    WT ge[9];
                                     ge[0] = ~WT(0);
                                     ge[1] = c0 | c1 | c2;
                                     ge[2] = c1 | c2;
@@ -323,7 +350,21 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                     WT cm0 = 0, cm1 = 0, cm2 = 0;
                                     uint32_t ct0 = 0, ct1 = 0, ct2 = 0;
                                     int ns = 0;
-                                    if (FLD) {
+                                    if (UZ && !FLD) {
+                                        // the step's uniform classes: masks by indexed selection
+                                        ns = un;
+#pragma unroll
+                                        for (int t = 0; t < 3; ++t) {
+                                            if (t < ns) {
+                                                const int ut = ut_s[t], us = us_s[t];
+                                                const WT eu = ut == 0 ? e0 : (ut == 1 ? e1 : e2);
+                                                const WT mk = eu & ge[us] & ~ge[us + 1];
+                                                if (t == 0) { cm0 = mk; ct0 = th_s[0]; }
+                                                else if (t == 1) { cm1 = mk; ct1 = th_s[1]; }
+                                                else { cm2 = mk; ct2 = th_s[2]; }
+                                            }
+                                        }
+                                    } else if (FLD) {
                                         ns = fc.ns;
 #pragma unroll
                                         for (int t = 0; t < 3; ++t) {
@@ -451,8 +492,48 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                             }
                             mine[i] = S_ ^ flips;
                         }
+                };
+                if constexpr (CACHED) {
+                    for (int c = 0; c < 2; ++c) {
+                        for (int k = 0; k < 2; ++k) {
+                            // (c, k) are uniform: plain selects among the cached registers
+                            const bool c0k = c == 0, k0k = k == 0;
+                            const int q = c0k ? (k0k ? cq[0][0] : cq[0][1]) : (k0k ? cq[1][0] : cq[1][1]);
+                            if (q < 0) continue;
+                            const int i = c0k ? (k0k ? cI[0][0] : cI[0][1]) : (k0k ? cI[1][0] : cI[1][1]);
+                            const uint32_t info = c0k ? (k0k ? cF[0][0] : cF[0][1]) : (k0k ? cF[1][0] : cF[1][1]);
+                            int nb[ZC];
+#pragma unroll
+                            for (int z = 0; z < ZC; ++z)
+                                nb[z] = c0k ? (k0k ? cN[0][0][z] : cN[0][1][z]) : (k0k ? cN[1][0][z] : cN[1][1][z]);
+                            spin(q, i, info, ZU, nb);
+                        }
+                        __syncthreads();
                     }
-                    __syncthreads();
+                } else {
+                    for (int c = 0; c < a.ncolors; ++c) {
+                        const int q0 = a.color_off[c], q1 = a.color_off[c + 1];
+                        for (int q = q0 + tid; q < q1; q += bd) {
+                            const int i = a.order[q];
+                            uint32_t info = 0;
+                            int zi = 0;
+                            int nb[8];
+                            if (PMJ) {
+                                info = a.cinfo[q];
+                                zi = ZU > 0 ? ZU : (UZ ? a.zuni : (int)((info >> 8) & 15));
+                                if (ZU > 0) {
+#pragma unroll
+                                    for (int z = 0; z < ZU; ++z) nb[z] = a.cnb[(size_t)z * N + q];
+                                } else {
+#pragma unroll
+                                    for (int z = 0; z < 8; ++z)
+                                        if (z < a.zmax) nb[z] = z < zi ? a.cnb[(size_t)z * N + q] : i;
+                                }
+                            }
+                            spin(q, i, info, zi, nb);
+                        }
+                        __syncthreads();
+                    }
                 }
             }
         }
@@ -487,12 +568,31 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
     for (int q = tid; q < 2 * N * nrep; q += bd) gst[q] = st[q];
 }
 
-template <typename WT, bool PMJ, bool UZ, bool FLD = false, int ZU = 0>
+template <typename WT, bool PMJ, bool UZ, bool FLD = false, int ZU = 0, bool CACHED = false>
 __global__ void colored_kernel(ColArgs<WT> a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int r0 = blockIdx.x * a.rpw;
-    colored_body<WT, PMJ, UZ, FLD, ZU>(a, r0, min(a.rpw, a.R - r0), smem);
+    colored_body<WT, PMJ, UZ, FLD, ZU, CACHED>(a, r0, min(a.rpw, a.R - r0), smem);
+}
+
+// Launch of the +-1 sweep for a uniform-degree lattice of compile-time degree
+// ZU, with per-thread cached neighbour lists when two colors of at most two
+// spins per thread allow it.
+template <typename WT, int ZU>
+void launch_uniform(const ColArgs<WT> &a, const piqmc_problem *pb, int grid, int block, size_t lds, hipStream_t st)
+{
+    int biggest = 0;
+    for (int c = 0; c < pb->ncolors; ++c) biggest = std::max(biggest, pb->color_off[c + 1] - pb->color_off[c]);
+    if (pb->ncolors == 2 && biggest <= 2 * block) {
+        (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true, false, ZU, true>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        colored_kernel<WT, true, true, false, ZU, true><<<grid, block, lds, st>>>(a);
+    } else {
+        (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true, false, ZU>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        colored_kernel<WT, true, true, false, ZU><<<grid, block, lds, st>>>(a);
+    }
 }
 
 // ------------------------------------------------------------------ dense path
@@ -776,11 +876,9 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         colored_kernel<WT, true, false, true><<<grid, block, lds, st>>>(a);
     } else if (pmj) {
         if (a.zuni == 4) {
-            (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true, false, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            colored_kernel<WT, true, true, false, 4><<<grid, block, lds, st>>>(a);
+            launch_uniform<WT, 4>(a, pb, grid, block, lds, st);
         } else if (a.zuni == 6) {
-            (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true, false, 6>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            colored_kernel<WT, true, true, false, 6><<<grid, block, lds, st>>>(a);
+            launch_uniform<WT, 6>(a, pb, grid, block, lds, st);
         } else if (a.zuni >= 0) {
             (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             colored_kernel<WT, true, true><<<grid, block, lds, st>>>(a);
