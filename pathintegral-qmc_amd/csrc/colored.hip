@@ -604,15 +604,36 @@ void launch_uniform(const ColArgs<WT> &a, const piqmc_problem *pb, int grid, int
 // order -- the CPU twin's arithmetic -- and the word's flips are gathered
 // with a ballot.  Stochastic lanes read their uniform from the same
 // bit-sliced Philox words as the word-per-thread path.
-template <typename WT>
-__device__ void dense_replica(const ColArgs<WT> &a, int r, WT *st)
+// LDS of one dense-path wavefront: its replica's packed state [2][N], then
+// (8-byte aligned) the staged row of the spin being updated -- couplings as
+// double [Z] and the neighbours' packed words [Z].
+__host__ __device__ inline size_t dense_wave_bytes(int N, int Z, int word_bytes)
 {
-    const int N = a.N, W = a.W, lane = threadIdx.x & 63;
+    const size_t state = ((size_t)2 * N * word_bytes + 7) & ~(size_t)7;
+    return state + (size_t)Z * 8 + (((size_t)Z * word_bytes + 7) & ~(size_t)7);
+}
+
+template <typename WT>
+__device__ void dense_replica(const ColArgs<WT> &a, int r, unsigned char *base)
+{
+    const int N = a.N, W = a.W, Z = a.Z, lane = threadIdx.x & 63;
+    WT *st = reinterpret_cast<WT *>(base);
+    double *jb = reinterpret_cast<double *>(base + (((size_t)2 * N * sizeof(WT) + 7) & ~(size_t)7));
+    WT *wb = reinterpret_cast<WT *>(jb + Z);
     const bool active = lane < W;
     const WT one = WT(1);
     const uint32_t rep = a.replica_base + (uint32_t)r;
     WT *gst = a.packed + (size_t)r * 2 * N;
     for (int q = lane; q < 2 * N; q += 64) st[q] = gst[q];
+    // a spin's row, one entry per lane (entries lane + 64k), read ahead of use;
+    // indices are clamped so the reads are unconditional
+    constexpr int kPre = 4;
+    int2 pre[kPre];
+    auto fetch_row = [&](int i) {
+        const int2 *row = a.rows + (size_t)i * Z;
+#pragma unroll
+        for (int k = 0; k < kPre; ++k) pre[k] = row[min(lane + 64 * k, Z - 1)];
+    };
     for (int s = 0; s < a.S; ++s) {
         const float jp = a.jperp[s];
         const double Ts = (double)a.temps[s];
@@ -621,68 +642,107 @@ __device__ void dense_replica(const ColArgs<WT> &a, int r, WT *st)
             for (int tau = 0; tau < 2; ++tau) {
                 WT *mine = st + tau * N;
                 const WT *other = st + (1 - tau) * N;
+                fetch_row(__builtin_amdgcn_readfirstlane(a.order[0]));
                 for (int q = 0; q < N; ++q) {  // colors in order; spins of one color are independent
                     const int i = __builtin_amdgcn_readfirstlane(a.order[q]);  // wave-uniform
+                    // stage the row: couplings and the current words of the neighbours
+                    // (a field entry stages word 0, so its term keeps the sign of J)
+#pragma unroll
+                    for (int k = 0; k < kPre; ++k) {
+                        const int e = lane + 64 * k;
+                        if (e < Z) {
+                            jb[e] = (double)__int_as_float(pre[k].y);
+                            wb[e] = pre[k].x == i ? WT(0) : mine[pre[k].x];
+                        }
+                    }
+                    for (int e = lane + 64 * kPre; e < Z; e += 64) {  // rows longer than 256
+                        const int2 rw = a.rows[(size_t)i * Z + e];
+                        jb[e] = (double)__int_as_float(rw.y);
+                        wb[e] = rw.x == i ? WT(0) : mine[rw.x];
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (q + 1 < N) fetch_row(__builtin_amdgcn_readfirstlane(a.order[q + 1]));
                     const WT S_ = mine[i], O = other[i];
                     const int lb = lane < W ? lane : 0;
                     // Trotter neighbours of slice 2lb+tau: slices 2lb+tau-1 and 2lb+tau+1
                     const int ml = tau == 0 ? (lb == 0 ? W - 1 : lb - 1) : lb;
                     const int mr = tau == 0 ? lb : (lb == W - 1 ? 0 : lb + 1);
                     const double si = ((S_ >> lb) & one) ? -1.0 : 1.0;
+                    // local field in row order (J*(+-1) is exact, so the select
+                    // form adds the same values as h += J*s_j)
+                    // J with its sign bit flipped by bit lb of w: shift that bit to
+                    // bit 31 and xor it into J's high word
+                    const uint32_t shl = 31u - (uint32_t)(lb & 31);
+                    auto term = [&](double J, WT w) __attribute__((always_inline)) {
+                        const uint32_t half = sizeof(WT) == 8 && lb >= 32 ? (uint32_t)((uint64_t)w >> 32) : (uint32_t)w;
+                        return __longlong_as_double(__double_as_longlong(J) ^
+                                                    ((long long)((half << shl) & 0x80000000u) << 32));
+                    };
                     double h = 0.0;
-                    const int2 *row = a.rows + (size_t)i * a.Z;  // uniform: scalar loads
                     int e = 0;
-                    // eight entries at a time: the row reads of the next group
-                    // are issued (clamped, so unconditionally) before the
-                    // spin-word reads and adds of this one; the adds keep row
-                    // order (J*(+-1) is exact, so the select form adds the same values)
-                    if (a.Z >= 8) {
-                        int2 rw[8];
+                    constexpr int G = 8;  // entries per group; the next group is read ahead
+                    if (Z >= G) {
+                        double jc[G];
+                        WT wc[G];
 #pragma unroll
-                        for (int k = 0; k < 8; ++k) rw[k] = row[k];
-                        for (; e + 8 <= a.Z; e += 8) {
-                            int2 nx[8];
-                            const int en = min(e + 8, a.Z - 8);
+                        for (int k = 0; k < G; ++k) {
+                            jc[k] = jb[k];
+                            wc[k] = wb[k];
+                        }
+                        for (; e + G <= Z; e += G) {
+                            const int en = min(e + G, Z - G);
+                            double jn[G];
+                            WT wn[G];
 #pragma unroll
-                            for (int k = 0; k < 8; ++k) nx[k] = row[en + k];
-                            WT wj[8];
-#pragma unroll
-                            for (int k = 0; k < 8; ++k) wj[k] = mine[rw[k].x];
-#pragma unroll
-                            for (int k = 0; k < 8; ++k) {
-                                const double J = (double)__int_as_float(rw[k].y);
-                                h += rw[k].x == i ? J : (((wj[k] >> lb) & one) ? -J : J);
+                            for (int k = 0; k < G; ++k) {
+                                jn[k] = jb[en + k];
+                                wn[k] = wb[en + k];
                             }
 #pragma unroll
-                            for (int k = 0; k < 8; ++k) rw[k] = nx[k];
+                            for (int k = 0; k < G; ++k) h += term(jc[k], wc[k]);
+#pragma unroll
+                            for (int k = 0; k < G; ++k) {
+                                jc[k] = jn[k];
+                                wc[k] = wn[k];
+                            }
                         }
                     }
-                    for (; e < a.Z; ++e) {
-                        const int2 rw = row[e];
-                        const double J = (double)__int_as_float(rw.y);
-                        h += rw.x == i ? J : (((mine[rw.x] >> lb) & one) ? -J : J);
-                    }
+                    for (; e < Z; ++e) h += term(jb[e], wb[e]);
                     const double ts = (((O >> ml) & one) ? -1.0 : 1.0) + (((O >> mr) & one) ? -1.0 : 1.0);
                     const double ed = -2.0 * si * (h + (double)jp * ts);
                     unsigned int thr = 0;
                     const int cls = piqmc_fixed_class(ed, Ts, &thr);
                     bool fl = active && cls == 1;
                     if (__ballot(active && cls == 2)) {
+                        // u < thr, compared most significant bit first: bit 31-b of
+                        // this lane's uniform is bit lb of random word b; a lane is
+                        // decided at the first bit where u and thr differ (equal
+                        // throughout means u == thr: rejected)
                         constexpr int kPer = sizeof(WT) == 4 ? 4 : 2;
-                        uint32_t u = 0;
-                        for (int qd = 0; qd < 32 / kPer; ++qd) {
+                        bool und = active && cls == 2, lt = false;
+                        for (int qd = 0; qd < 32 / kPer && __ballot(und); ++qd) {
                             uint32_t out[4];
                             philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd, a.key0,
                                           a.key1, out);
 #pragma unroll
-                            for (int t = 0; t < kPer; ++t)
-                                u |= (uint32_t)((rword<WT>(out, t) >> lb) & one) << (31 - (qd * kPer + t));
+                            for (int t = 0; t < kPer; ++t) {
+                                const uint32_t ub = (uint32_t)((rword<WT>(out, t) >> lb) & one);
+                                const uint32_t tb = (thr >> (31 - (qd * kPer + t))) & 1u;
+                                if (und && ub != tb) {
+                                    lt = ub < tb;
+                                    und = false;
+                                }
+                            }
                         }
-                        if (active && cls == 2) fl = u < thr;
+                        if (active && cls == 2) fl = lt;
                     }
                     const unsigned long long bal = __ballot(fl);
                     if (lane == 0) mine[i] = S_ ^ (WT)bal;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
             }
         }
@@ -713,7 +773,7 @@ __global__ void dense_kernel(ColArgs<WT> a)
     extern __shared__ __align__(16) unsigned char smem[];
     const int w = threadIdx.x >> 6, r = blockIdx.x * (blockDim.x >> 6) + w;
     if (r >= a.R) return;
-    dense_replica<WT>(a, r, reinterpret_cast<WT *>(smem) + (size_t)w * 2 * a.N);
+    dense_replica<WT>(a, r, smem + (size_t)w * dense_wave_bytes(a.N, a.Z, sizeof(WT)));
 }
 
 // int8 [R][P][N] -> packed [R][2][N] (bit m of parity tau = slice 2m+tau, set = -1)
@@ -802,10 +862,12 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
     const int N = pb->N;
     // dense path when color classes average fewer than 8 spins (or forced)
     if (use_dense(pb)) {
-        if (block <= 0) block = 256;
+        // one replica per wavefront; small replica counts get one wavefront per
+        // workgroup so that they spread over the CUs
+        if (block <= 0) block = R >= 4 * 256 ? 256 : (R >= 2 * 256 ? 128 : 64);
         if (block > 1024 || (block & 63)) return piqmc::set_error(PIQMC_EINVAL, "block must be a multiple of 64, <= 1024");
         const int wpb = block / 64;
-        const size_t lds = sizeof(WT) * 2 * (size_t)N * wpb;
+        const size_t lds = dense_wave_bytes(N, pb->Z, sizeof(WT)) * wpb;
         if (lds > 160 * 1024) return piqmc::set_error(PIQMC_EINVAL, "dense sweep: state exceeds LDS");
         float *d_jp = nullptr;  // [S] J_perp, then [S] temperatures
         WT *d_packed = nullptr;
