@@ -190,6 +190,14 @@ int piqmc_batch_add(piqmc_batch *b, const piqmc_problem *pb, int R, int P, int S
 /* One launch over every job; block: threads per workgroup (0 = 256), rpw:
  * replicas per work item for word-per-thread jobs (0 = automatic). */
 int piqmc_batch_run(piqmc_batch *b, int block, int rpw, void *stream);
+/* piqmc_batch_run in two parts.  prepare: tables, packed-state buffers and
+ * the sorted work queue on the device (owned by the batch until the next
+ * prepare or destroy).  launch: pack every job's spins, the persistent
+ * work-queue kernel, unpack -- kernel launches only, so a stream capture of
+ * it gives a HIP graph that re-runs the whole batch (each replay continues
+ * from the spins the previous one left). */
+int piqmc_batch_prepare(piqmc_batch *b, int block, int rpw, void *stream);
+int piqmc_batch_launch(piqmc_batch *b, void *stream);
 int piqmc_batch_info(const piqmc_batch *b, int *njobs, int *nitems, int *grid);
 
 /* ------------------------------------------------------------------ */

@@ -64,6 +64,8 @@ _SIGS = {
     "piqmc_batch_destroy": ([_vp], _i),
     "piqmc_batch_add": ([_vp, _vp, _i, _i, _i, _i, _f, _vp, _vp, _u64, _u32, _u32, _vp], _i),
     "piqmc_batch_run": ([_vp, _i, _i, _vp], _i),
+    "piqmc_batch_prepare": ([_vp, _i, _i, _vp], _i),
+    "piqmc_batch_launch": ([_vp, _vp], _i),
     "piqmc_batch_info": ([_vp, _vp, _vp, _vp], _i),
 }
 for _name, (_args, _res) in _SIGS.items():

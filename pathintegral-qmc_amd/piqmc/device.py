@@ -252,6 +252,26 @@ class Batch:
     def run(self, block=0, rpw=0, stream=None):
         _lib.call("piqmc_batch_run", self._h, int(block), int(rpw), _stream(stream))
 
+    def prepare(self, block=0, rpw=0, stream=None):
+        """Upload the tables and build the work queue (then ``launch``)."""
+        _lib.call("piqmc_batch_prepare", self._h, int(block), int(rpw), _stream(stream))
+
+    def launch(self, stream=None):
+        """Pack, sweep every job, unpack: kernel launches only."""
+        _lib.call("piqmc_batch_launch", self._h, _stream(stream))
+
+    def capture(self, block=0, rpw=0):
+        """A torch CUDAGraph (a HIP graph) of ``launch``: each replay runs every
+        job's schedule again from the spins the previous run left."""
+        import torch
+        self.prepare(block, rpw)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()
+        with torch.cuda.graph(g, stream=side):
+            self.launch(stream=side)
+        return g
+
     def info(self):
         v = [C.c_int() for _ in range(3)]
         _lib.call("piqmc_batch_info", self._h, *[C.byref(x) for x in v])

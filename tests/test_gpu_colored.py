@@ -230,3 +230,41 @@ def test_batch_heterogeneous_equals_individual_runs(gpu):
         torch.cuda.synchronize()
         assert torch.equal(c8, ref)
         torch.testing.assert_close(et, eref, rtol=0, atol=1e-12)
+
+
+def test_batch_graph_replays_match_eager(gpu):
+    # the batch launch captured once into a HIP graph; every replay (with
+    # eager work and synchronisation in between) equals an eager run from the
+    # same spins
+    import torch
+    from piqmc import device, instances
+    specs = [("pm", 8, 2, 16, 20, 4), ("field", 4, 3, 32, 12, 3), ("sk", 40, 0, 16, 6, 4)]
+    rng = np.random.RandomState(5)
+    jobs = []
+    for t, (kind, L, dim, P, S, R) in enumerate(specs):
+        if kind == "sk":
+            n, r, c, v = instances.sk(L, np.random.RandomState(t))
+        else:
+            n, r, c, v = instances.lattice_pm_j(L, dim, np.random.RandomState(t),
+                                                 0.1 if kind == "field" else 0.0)
+        pb = device.Problem(instances.neighbors(n, r, c, v))
+        c0 = (2 * rng.randint(2, size=(R, P, n)) - 1).astype(np.int8)
+        jobs.append((pb, c0, np.linspace(3.0, 1e-8, S), 0.05, 300 + t))
+    bg, be = device.Batch(), device.Batch()
+    cg, ce = [], []
+    for pb, c0, sched, T, seed in jobs:
+        a = torch.tensor(c0, device=gpu)
+        e = torch.tensor(c0, device=gpu)
+        bg.add(pb, a, sched, 1, T, seed)
+        be.add(pb, e, sched, 1, T, seed)
+        cg.append(a)
+        ce.append(e)
+    graph = bg.capture()
+    for _ in range(3):
+        graph.replay()
+        torch.cuda.synchronize()
+        be.run()
+        torch.cuda.synchronize()
+        for a, e in zip(cg, ce):
+            assert torch.equal(a, e)
+    assert not all(torch.equal(a, torch.tensor(c0, device=gpu)) for a, (_, c0, _, _, _) in zip(cg, jobs))
