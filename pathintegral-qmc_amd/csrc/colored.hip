@@ -24,12 +24,25 @@
 #include <math.h>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "piqmc_internal.h"
 
 namespace {
 
 constexpr int kClassStride = 8 * 8 * 3;  // (z, u_s, u_t) thresholds per field step
+// Stochastic-word queue of the +-1 sweep: records (index | replica << 16, three
+// class masks) per wavefront; it never holds more than 63 + 64 between drains.
+constexpr int kQueueCap = 128;
+
+// LDS of the word-per-thread sweep: packed state [rpw][2][N] words, class
+// thresholds [kClassStride], per-degree table [8], reduction scratch [16]
+// doubles, then (16-byte aligned) the wavefronts' stochastic queues.
+__host__ __device__ inline size_t queue_offset(int N, int rpw, int word_bytes)
+{
+    return ((size_t)word_bytes * 2 * (size_t)N * rpw + 4 * (kClassStride + 8) + 16 * sizeof(double) + 15) &
+           ~(size_t)15;
+}
 
 template <typename WT>
 struct ColArgs {
@@ -199,19 +212,24 @@ __device__ __forceinline__ void field_classes(double h, int z, float jp, double 
 // CACHED (with ZU > 0): two colors of at most two spins per thread each; every
 // thread keeps its spins' indices and neighbour lists in registers for the
 // whole schedule instead of reloading them in every phase.
-template <typename WT, bool PMJ, bool UZ = false, bool FLD = false, int ZU = 0, bool CACHED = false>
+template <typename WT, bool PMJ, bool UZ = false, bool FLD = false, int ZU = 0, bool CACHED = false, bool CQ = false>
 __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned char *smem)
 {
+    static_assert(!CQ || (UZ && PMJ && !FLD && sizeof(WT) == 4), "stochastic queue: +-1 uniform-degree 32-bit path");
     const int N = a.N, W = a.W, tid = threadIdx.x, bd = blockDim.x;
     WT *st = reinterpret_cast<WT *>(smem);                            // [nrep][2][N]
     uint32_t *tthr = reinterpret_cast<uint32_t *>(st + (size_t)2 * N * a.rpw);  // [kClassStride]
     uint32_t *tz = tthr + kClassStride;                               // [8]
     double *red = reinterpret_cast<double *>(tz + 8);                 // [16]
+    // per-wavefront queue of stochastic words (CQ): [waves][kQueueCap]
+    uint4 *wq = reinterpret_cast<uint4 *>(smem + queue_offset(a.N, a.rpw, sizeof(WT))) +
+                (size_t)(tid >> 6) * kQueueCap;
+    int qn = 0;  // queued records of this wavefront (wave-uniform)
     const WT valid = (W == (int)(8 * sizeof(WT))) ? ~WT(0) : ((WT(1) << W) - 1);
     WT *gst = a.packed + (size_t)r0 * 2 * N;
     for (int q = tid; q < 2 * N * nrep; q += bd) st[q] = gst[q];
     constexpr int ZC = ZU > 0 ? ZU : 1;
-    int cq[2][2], cI[2][2], cN[2][2][ZC];
+    int cq_[2][2], cI[2][2], cN[2][2][ZC];
     uint32_t cF[2][2];
     if (CACHED) {
 #pragma unroll
@@ -221,7 +239,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                 const int q = a.color_off[c] + tid + k * bd;
                 const bool in = q < a.color_off[c + 1];
                 const int qs = in ? q : 0;
-                cq[c][k] = in ? q : -1;
+                cq_[c][k] = in ? q : -1;
                 cI[c][k] = a.order[qs];
                 cF[c][k] = a.cinfo[qs];
 #pragma unroll
@@ -263,12 +281,46 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
         }
         const float jp = a.jperp[s];
         const double Ts = (double)a.temps[s];
+        const bool cq = CQ && un <= 3;  // this step's stochastic words go through the queue
         for (int m = 0; m < a.M; ++m) {
             const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
             for (int tau = 0; tau < 2; ++tau) {
+                // The top cnt queued words, one per lane: each word's stochastic
+                // bits meet the same bit-sliced uniforms, thresholds and early
+                // exit as in the inline path, and the accepted ones are XORed
+                // into the word (its deterministic flips are already stored).
+                auto drain = [&](const int cnt) __attribute__((always_inline)) {
+                    __builtin_amdgcn_wave_barrier();
+                    const int lane = tid & 63;
+                    if (lane < cnt) {
+                        const uint4 rec = wq[qn - cnt + lane];
+                        const int i = (int)(rec.x & 0xffffu), rr = (int)(rec.x >> 16);
+                        const WT m0 = (WT)rec.y, m1 = (WT)rec.z, m2 = (WT)rec.w;
+                        const uint32_t rep = a.replica_base + (uint32_t)(r0 + rr);
+                        WT lt = 0, und = m0 | m1 | m2;
+                        for (int qd = 0; qd < 8 && und; ++qd) {
+                            uint32_t out[4];
+                            philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd, a.key0, a.key1, out);
+#pragma unroll
+                            for (int t = 0; t < 4; ++t) {
+                                const WT Rb = rword<WT>(out, t);
+                                const int sh = 31 - (qd * 4 + t);
+                                const WT tb = (((th_s[0] >> sh) & 1u) ? m0 : WT(0)) |
+                                              (((th_s[1] >> sh) & 1u) ? m1 : WT(0)) |
+                                              (((th_s[2] >> sh) & 1u) ? m2 : WT(0));
+                                lt |= und & ~Rb & tb;
+                                und &= ~(Rb ^ tb);
+                            }
+                        }
+                        st[((size_t)rr * 2 + tau) * N + i] ^= lt;
+                    }
+                    qn -= cnt;
+                    __builtin_amdgcn_wave_barrier();
+                };
                 // one spin (position q in color order, index i, neighbours nb)
-                // for every replica of the workgroup
-                auto spin = [&](const int q, const int i, const uint32_t info, const int zi,
+                // for every replica of the workgroup; lanes with act false take
+                // part only in the queue's wave-wide steps
+                auto spin = [&](const bool act, const int q, const int i, const uint32_t info, const int zi,
                                 const int *nb) __attribute__((always_inline)) {
                         const uint32_t zt = UZ ? zt_u : ((PMJ && !FLD) ? tz[zi] : 0u);
                         // fields: this spin's class thresholds, shared by the replicas below
@@ -279,6 +331,8 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                             field_classes(hfield, zi, jp, Ts, fc);
                         }
                         for (int rr = 0; rr < nrep; ++rr) {
+                          WT qX = 0, qm0 = 0, qm1 = 0, qm2 = 0;  // stochastic word for the queue
+                          if (act) {
                             WT *mine = st + ((size_t)rr * 2 + tau) * N;
                             const WT *other = st + ((size_t)rr * 2 + 1 - tau) * N;
                             const uint32_t rep = a.replica_base + (uint32_t)(r0 + rr);
@@ -338,7 +392,21 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 }
                                 flips &= valid;
                                 X &= valid & ~flips;
-                                if (X) {
+                                if (CQ && cq) {
+                                    // masks of the step's uniform classes, queued with the word
+                                    qX = X;
+#pragma unroll
+                                    for (int t = 0; t < 3; ++t) {
+                                        if (t < un) {
+                                            const int ut = ut_s[t], us = us_s[t];
+                                            const WT eu = ut == 0 ? e0 : (ut == 1 ? e1 : e2);
+                                            const WT mk = X & eu & ge[us] & ~ge[us + 1];
+                                            if (t == 0) qm0 = mk;
+                                            else if (t == 1) qm1 = mk;
+                                            else qm2 = mk;
+                                        }
+                                    }
+                                } else if (X) {
                                     // Bit-sliced comparison u < thr for every stochastic bit at
                                     // once, most significant uniform bit first: random word R_b
                                     // supplies bit 31-b of all uniforms; a bit is decided at the
@@ -491,6 +559,23 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 }
                             }
                             mine[i] = S_ ^ flips;
+                          }
+                          if constexpr (CQ) {
+                              if (cq) {
+                                  // append this lane's stochastic word (if any) to the
+                                  // wavefront's queue; a full batch of 64 is drained at once
+                                  const unsigned long long bal = __ballot(qX != 0);
+                                  if (qX != 0) {
+                                      const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi(
+                                                               (uint32_t)(bal >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                                      wq[pos] = make_uint4((uint32_t)i | ((uint32_t)rr << 16), (uint32_t)qm0,
+                                                           (uint32_t)qm1, (uint32_t)qm2);
+                                  }
+                                  qn += (int)__popcll(bal);
+                                  if (qn >= 64) drain(64);
+                              }
+                          }
                         }
                 };
                 if constexpr (CACHED) {
@@ -498,22 +583,27 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                         for (int k = 0; k < 2; ++k) {
                             // (c, k) are uniform: plain selects among the cached registers
                             const bool c0k = c == 0, k0k = k == 0;
-                            const int q = c0k ? (k0k ? cq[0][0] : cq[0][1]) : (k0k ? cq[1][0] : cq[1][1]);
-                            if (q < 0) continue;
+                            const int q = c0k ? (k0k ? cq_[0][0] : cq_[0][1]) : (k0k ? cq_[1][0] : cq_[1][1]);
+                            if (!CQ && q < 0) continue;
                             const int i = c0k ? (k0k ? cI[0][0] : cI[0][1]) : (k0k ? cI[1][0] : cI[1][1]);
                             const uint32_t info = c0k ? (k0k ? cF[0][0] : cF[0][1]) : (k0k ? cF[1][0] : cF[1][1]);
                             int nb[ZC];
 #pragma unroll
                             for (int z = 0; z < ZC; ++z)
                                 nb[z] = c0k ? (k0k ? cN[0][0][z] : cN[0][1][z]) : (k0k ? cN[1][0][z] : cN[1][1][z]);
-                            spin(q, i, info, ZU, nb);
+                            spin(q >= 0, q, i, info, ZU, nb);
                         }
+                        if (CQ && qn > 0) drain(qn);
                         __syncthreads();
                     }
                 } else {
                     for (int c = 0; c < a.ncolors; ++c) {
                         const int q0 = a.color_off[c], q1 = a.color_off[c + 1];
-                        for (int q = q0 + tid; q < q1; q += bd) {
+                        for (int qb = q0; qb < q1; qb += bd) {
+                            // with the queue every lane runs every pass (inactive past q1)
+                            const bool act = qb + tid < q1;
+                            if (!CQ && !act) break;
+                            const int q = act ? qb + tid : q0;
                             const int i = a.order[q];
                             uint32_t info = 0;
                             int zi = 0;
@@ -530,8 +620,9 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                         if (z < a.zmax) nb[z] = z < zi ? a.cnb[(size_t)z * N + q] : i;
                                 }
                             }
-                            spin(q, i, info, zi, nb);
+                            spin(act, q, i, info, zi, nb);
                         }
+                        if (CQ && qn > 0) drain(qn);
                         __syncthreads();
                     }
                 }
@@ -568,31 +659,60 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
     for (int q = tid; q < 2 * N * nrep; q += bd) gst[q] = st[q];
 }
 
-template <typename WT, bool PMJ, bool UZ, bool FLD = false, int ZU = 0, bool CACHED = false>
+template <typename WT, bool PMJ, bool UZ, bool FLD = false, int ZU = 0, bool CACHED = false, bool CQ = false>
 __global__ void colored_kernel(ColArgs<WT> a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int r0 = blockIdx.x * a.rpw;
-    colored_body<WT, PMJ, UZ, FLD, ZU, CACHED>(a, r0, min(a.rpw, a.R - r0), smem);
+    colored_body<WT, PMJ, UZ, FLD, ZU, CACHED, CQ>(a, r0, min(a.rpw, a.R - r0), smem);
 }
 
 // Launch of the +-1 sweep for a uniform-degree lattice of compile-time degree
 // ZU, with per-thread cached neighbour lists when two colors of at most two
-// spins per thread allow it.
+// spins per thread allow it, and with the per-wavefront stochastic queue when
+// its LDS fits (32-bit words).
+template <typename WT, int ZU, bool CQ>
+void launch_uniform_q(const ColArgs<WT> &a, bool cached, int grid, int block, size_t lds, hipStream_t st)
+{
+    if (cached) {
+        (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true, false, ZU, true, CQ>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        colored_kernel<WT, true, true, false, ZU, true, CQ><<<grid, block, lds, st>>>(a);
+    } else {
+        (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true, false, ZU, false, CQ>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        colored_kernel<WT, true, true, false, ZU, false, CQ><<<grid, block, lds, st>>>(a);
+    }
+}
+
+// The stochastic queue is on unless PIQMC_STOCHASTIC_QUEUE=0 (both forms give
+// identical results; the switch exists to compare them).
+inline bool queue_enabled()
+{
+    const char *e = std::getenv("PIQMC_STOCHASTIC_QUEUE");
+    return !(e && e[0] == '0');
+}
+
+// LDS of the word-per-thread sweep with the stochastic queues of block/64 wavefronts.
+inline size_t queue_lds(int N, int rpw, int word_bytes, int block)
+{
+    return queue_offset(N, rpw, word_bytes) + (size_t)(block / 64) * kQueueCap * sizeof(uint4);
+}
+
 template <typename WT, int ZU>
 void launch_uniform(const ColArgs<WT> &a, const piqmc_problem *pb, int grid, int block, size_t lds, hipStream_t st)
 {
     int biggest = 0;
     for (int c = 0; c < pb->ncolors; ++c) biggest = std::max(biggest, pb->color_off[c + 1] - pb->color_off[c]);
-    if (pb->ncolors == 2 && biggest <= 2 * block) {
-        (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true, false, ZU, true>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        colored_kernel<WT, true, true, false, ZU, true><<<grid, block, lds, st>>>(a);
-    } else {
-        (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true, false, ZU>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        colored_kernel<WT, true, true, false, ZU><<<grid, block, lds, st>>>(a);
+    const bool cached = pb->ncolors == 2 && biggest <= 2 * block;
+    const size_t lq = queue_lds(a.N, a.rpw, sizeof(WT), block);
+    if constexpr (sizeof(WT) == 4) {
+        if (lq <= 160 * 1024 && queue_enabled()) {
+            launch_uniform_q<WT, ZU, true>(a, cached, grid, block, lq, st);
+            return;
+        }
     }
+    launch_uniform_q<WT, ZU, false>(a, cached, grid, block, lds, st);
 }
 
 // ------------------------------------------------------------------ dense path
@@ -942,8 +1062,15 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         } else if (a.zuni == 6) {
             launch_uniform<WT, 6>(a, pb, grid, block, lds, st);
         } else if (a.zuni >= 0) {
-            (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            colored_kernel<WT, true, true><<<grid, block, lds, st>>>(a);
+            const size_t lq = queue_lds(N, rpw, sizeof(WT), block);
+            if (sizeof(WT) == 4 && lq <= 160 * 1024 && queue_enabled()) {
+                (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true, false, 0, false, sizeof(WT) == 4>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lq);
+                colored_kernel<WT, true, true, false, 0, false, sizeof(WT) == 4><<<grid, block, lq, st>>>(a);
+            } else {
+                (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                colored_kernel<WT, true, true><<<grid, block, lds, st>>>(a);
+            }
         } else {
             (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             colored_kernel<WT, true, false><<<grid, block, lds, st>>>(a);

@@ -268,3 +268,23 @@ def test_batch_graph_replays_match_eager(gpu):
         for a, e in zip(cg, ce):
             assert torch.equal(a, e)
     assert not all(torch.equal(a, torch.tensor(c0, device=gpu)) for a, (_, c0, _, _, _) in zip(cg, jobs))
+
+
+@pytest.mark.parametrize("queue", ["1", "0"])
+@pytest.mark.parametrize("L,dim,P,T,S,block,rpw", [
+    (32, 2, 64, 1 / 64, 40, 64, 1),     # word loop per color (no cached neighbours)
+    (16, 2, 32, 1 / 32, 30, 256, 2),    # cached neighbours, degree 4
+    (64, 1, 16, 0.1, 30, 128, 3),       # ring: uniform degree 2 (run-time degree)
+    (4, 3, 12, 0.1, 20, 1024, 2),       # degree 6, one pass per color with idle lanes
+    (8, 2, 4, 0.5, 20, 64, 1),          # many stochastic classes (inline path)
+])
+def test_colored_stochastic_queue(gpu, monkeypatch, queue, L, dim, P, T, S, block, rpw):
+    # stochastic words drained from the per-wavefront queue, or resolved in
+    # place (PIQMC_STOCHASTIC_QUEUE=0), give the oracle's result bit for bit
+    monkeypatch.setenv("PIQMC_STOCHASTIC_QUEUE", queue)
+    N, nbs, _, _, _ = pm_j_problem(L, dim=dim, seed=L + P)
+    pb, got, ref, etg, etr = _run(gpu, nbs, P, np.linspace(3.0, 1e-8, S), 1, T, R=3, seed=31 + P,
+                                  rbase=1, sbase=2, block=block, rpw=rpw)
+    assert pb.pm_j
+    assert np.array_equal(got, ref)
+    assert np.array_equal(etg, etr)
