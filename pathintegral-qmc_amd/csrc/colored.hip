@@ -62,16 +62,6 @@ struct ColArgs {
     double *etrace;            // [R][S] or null
 };
 
-template <typename WT>
-__device__ __forceinline__ WT rotl1(WT x, int W, WT valid)
-{
-    return ((x << 1) | (x >> (W - 1))) & valid;
-}
-template <typename WT>
-__device__ __forceinline__ WT rotr1(WT x, int W)
-{
-    return (x >> 1) | ((x & WT(1)) << (W - 1));
-}
 __device__ __forceinline__ int popc(uint32_t x) { return __popc(x); }
 __device__ __forceinline__ int popc(uint64_t x) { return __popcll(x); }
 __device__ __forceinline__ int ctz(uint32_t x) { return __ffs(x) - 1; }
@@ -285,6 +275,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
         for (int m = 0; m < a.M; ++m) {
             const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
             for (int tau = 0; tau < 2; ++tau) {
+                const int tsa = tau == 0 ? 1 : W - 1, tsb = tau == 0 ? W - 1 : 1;  // rotation by one slice
                 // The top cnt queued words, one per lane: each word's stochastic
                 // bits meet the same bit-sliced uniforms, thresholds and early
                 // exit as in the inline path, and the accepted ones are XORed
@@ -337,8 +328,10 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                             const WT *other = st + ((size_t)rr * 2 + 1 - tau) * N;
                             const uint32_t rep = a.replica_base + (uint32_t)(r0 + rr);
                             const WT S_ = mine[i], O = other[i];
-                            const WT Lw = tau == 0 ? rotl1(O, W, valid) : O;
-                            const WT Rw = tau == 0 ? O : rotr1(O, W);
+                            // Trotter neighbours of slice 2m+tau: bits m of O and of O
+                            // rotated by one slice (left for tau 0, right for tau 1);
+                            // everything below is symmetric in the two
+                            const WT Lw = O, Rw = ((O << tsa) | (O >> tsb)) & valid;
                             const WT uL = S_ ^ Lw, uR = S_ ^ Rw;  // 1 = antiparallel Trotter bond
                             WT flips = 0;
                             if (PMJ) {
