@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 #include "piqmc_internal.h"
 
@@ -280,13 +281,14 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                 // bits meet the same bit-sliced uniforms, thresholds and early
                 // exit as in the inline path, and the accepted ones are XORed
                 // into the word (its deterministic flips are already stored).
-                auto drain = [&](const int cnt) __attribute__((always_inline)) {
+                // NS: classes of the step (1, or up to 3), a compile-time constant.
+                auto drain = [&](const int cnt, auto NS) __attribute__((always_inline)) {
                     __builtin_amdgcn_wave_barrier();
                     const int lane = tid & 63;
                     if (lane < cnt) {
                         const uint4 rec = wq[qn - cnt + lane];
                         const int i = (int)(rec.x & 0xffffu), rr = (int)(rec.x >> 16);
-                        const WT m0 = (WT)rec.y, m1 = (WT)rec.z, m2 = (WT)rec.w;
+                        const WT m0 = (WT)rec.y, m1 = NS() > 1 ? (WT)rec.z : WT(0), m2 = NS() > 2 ? (WT)rec.w : WT(0);
                         const uint32_t rep = a.replica_base + (uint32_t)(r0 + rr);
                         WT lt = 0, und = m0 | m1 | m2;
                         for (int qd = 0; qd < 8 && und; ++qd) {
@@ -387,7 +389,10 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 X &= valid & ~flips;
                                 if (CQ && cq) {
                                     // masks of the step's uniform classes, queued with the word
+                                    // (a single class holds every stochastic bit)
                                     qX = X;
+                                    if (un == 1) qm0 = X;
+                                    else
 #pragma unroll
                                     for (int t = 0; t < 3; ++t) {
                                         if (t < un) {
@@ -566,7 +571,10 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                                            (uint32_t)qm1, (uint32_t)qm2);
                                   }
                                   qn += (int)__popcll(bal);
-                                  if (qn >= 64) drain(64);
+                                  if (qn >= 64) {
+                                      if (un == 1) drain(64, std::integral_constant<int, 1>());
+                                      else drain(64, std::integral_constant<int, 3>());
+                                  }
                               }
                           }
                         }
@@ -586,7 +594,10 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 nb[z] = c0k ? (k0k ? cN[0][0][z] : cN[0][1][z]) : (k0k ? cN[1][0][z] : cN[1][1][z]);
                             spin(q >= 0, q, i, info, ZU, nb);
                         }
-                        if (CQ && qn > 0) drain(qn);
+                        if (CQ && qn > 0) {
+                            if (un == 1) drain(qn, std::integral_constant<int, 1>());
+                            else drain(qn, std::integral_constant<int, 3>());
+                        }
                         __syncthreads();
                     }
                 } else {
@@ -615,7 +626,10 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                             }
                             spin(act, q, i, info, zi, nb);
                         }
-                        if (CQ && qn > 0) drain(qn);
+                        if (CQ && qn > 0) {
+                            if (un == 1) drain(qn, std::integral_constant<int, 1>());
+                            else drain(qn, std::integral_constant<int, 3>());
+                        }
                         __syncthreads();
                     }
                 }
