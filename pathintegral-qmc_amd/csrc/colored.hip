@@ -204,7 +204,7 @@ __device__ __forceinline__ void field_classes(double h, int z, float jp, double 
 // thread keeps its spins' indices and neighbour lists in registers for the
 // whole schedule instead of reloading them in every phase.
 template <typename WT, bool PMJ, bool UZ = false, bool FLD = false, int ZU = 0, bool CACHED = false, bool CQ = false>
-__device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned char *smem)
+__device__ __forceinline__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned char *smem)
 {
     static_assert(!CQ || (UZ && PMJ && !FLD && sizeof(WT) == 4), "stochastic queue: +-1 uniform-degree 32-bit path");
     const int N = a.N, W = a.W, tid = threadIdx.x, bd = blockDim.x;
@@ -286,7 +286,13 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                     __builtin_amdgcn_wave_barrier();
                     const int lane = tid & 63;
                     if (lane < cnt) {
-                        const uint4 rec = wq[qn - cnt + lane];
+                        uint4 rec;  // single-class records are (index | replica << 16, mask)
+                        if constexpr (NS() == 1) {
+                            const uint2 r2 = *reinterpret_cast<const uint2 *>(wq + (qn - cnt + lane));
+                            rec = make_uint4(r2.x, r2.y, 0u, 0u);
+                        } else {
+                            rec = wq[qn - cnt + lane];
+                        }
                         const int i = (int)(rec.x & 0xffffu), rr = (int)(rec.x >> 16);
                         const WT m0 = (WT)rec.y, m1 = NS() > 1 ? (WT)rec.z : WT(0), m2 = NS() > 2 ? (WT)rec.w : WT(0);
                         const uint32_t rep = a.replica_base + (uint32_t)(r0 + rr);
@@ -349,7 +355,8 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                         c2 ^= k1;
                                     }
                                 }
-                                const WT e0 = ~(uL | uR), e1 = uL ^ uR, e2 = uL & uR;
+                                // Trotter classes; bits past W are zero in every class
+                                const WT e0 = ~(uL | uR) & valid, e1 = uL ^ uR, e2 = uL & uR;
                                 WT X;
                                 WT ge[9];  // "count >= k" masks (uniform-degree path)
                                 if (FLD) {
@@ -385,8 +392,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                         (e1 & ge_pick((int)((zt >> 12) & 15), c0, c1, c2)) |
                                         (e2 & ge_pick((int)((zt >> 20) & 15), c0, c1, c2));
                                 }
-                                flips &= valid;
-                                X &= valid & ~flips;
+                                X &= ~flips;
                                 if (CQ && cq) {
                                     // masks of the step's uniform classes, queued with the word
                                     // (a single class holds every stochastic bit)
@@ -567,8 +573,12 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                       const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi(
                                                                (uint32_t)(bal >> 32),
                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                                      wq[pos] = make_uint4((uint32_t)i | ((uint32_t)rr << 16), (uint32_t)qm0,
-                                                           (uint32_t)qm1, (uint32_t)qm2);
+                                      if (un == 1)
+                                          *reinterpret_cast<uint2 *>(wq + pos) =
+                                              make_uint2((uint32_t)i | ((uint32_t)rr << 16), (uint32_t)qm0);
+                                      else
+                                          wq[pos] = make_uint4((uint32_t)i | ((uint32_t)rr << 16), (uint32_t)qm0,
+                                                               (uint32_t)qm1, (uint32_t)qm2);
                                   }
                                   qn += (int)__popcll(bal);
                                   if (qn >= 64) {
@@ -580,9 +590,11 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                         }
                 };
                 if constexpr (CACHED) {
+#pragma unroll
                     for (int c = 0; c < 2; ++c) {
+#pragma unroll
                         for (int k = 0; k < 2; ++k) {
-                            // (c, k) are uniform: plain selects among the cached registers
+                            // (c, k) are compile-time after unrolling: the cached registers directly
                             const bool c0k = c == 0, k0k = k == 0;
                             const int q = c0k ? (k0k ? cq_[0][0] : cq_[0][1]) : (k0k ? cq_[1][0] : cq_[1][1]);
                             if (!CQ && q < 0) continue;
