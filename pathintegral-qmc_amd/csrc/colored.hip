@@ -204,7 +204,7 @@ __device__ __forceinline__ void field_classes(double h, int z, float jp, double 
 // thread keeps its spins' indices and neighbour lists in registers for the
 // whole schedule instead of reloading them in every phase.
 template <typename WT, bool PMJ, bool UZ = false, bool FLD = false, int ZU = 0, bool CACHED = false, bool CQ = false>
-__device__ __forceinline__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned char *smem)
+__device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned char *smem)
 {
     static_assert(!CQ || (UZ && PMJ && !FLD && sizeof(WT) == 4), "stochastic queue: +-1 uniform-degree 32-bit path");
     const int N = a.N, W = a.W, tid = threadIdx.x, bd = blockDim.x;
@@ -678,8 +678,10 @@ __device__ __forceinline__ void colored_body(const ColArgs<WT> &a, int r0, int n
     for (int q = tid; q < 2 * N * nrep; q += bd) gst[q] = st[q];
 }
 
+// flatten: the sweep body is inlined whole (its unrolled phases need the
+// cached registers of the kernel's own frame)
 template <typename WT, bool PMJ, bool UZ, bool FLD = false, int ZU = 0, bool CACHED = false, bool CQ = false>
-__global__ void colored_kernel(ColArgs<WT> a)
+__global__ void __attribute__((flatten)) colored_kernel(ColArgs<WT> a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int r0 = blockIdx.x * a.rpw;
