@@ -949,10 +949,95 @@ __global__ void unpack_parity_kernel(const WT *packed, int8_t *confs, int R, int
     }
 }
 
+// The same two layouts for 16-spin groups: one thread owns spins 16q..16q+15
+// of one replica for all P slices, so every slice row moves as one 16-byte
+// access per lane (a wavefront covers 1 KiB contiguous) and the packed words
+// are read or written once.  Needs N % 16 == 0 and 16-byte aligned spins.
+template <typename WT>
+__global__ void pack_parity16_kernel(const int8_t *confs, WT *packed, int R, int P, int N)
+{
+    const int G = N / 16;
+    const size_t tot = (size_t)R * G;
+    for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < tot;
+         x += (size_t)gridDim.x * blockDim.x) {
+        const int q = (int)(x % G), r = (int)(x / G);
+        const int8_t *src = confs + (size_t)r * P * N + 16 * (size_t)q;
+        WT w0[16], w1[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) w0[j] = w1[j] = 0;
+        for (int k = 0; k < P; k += 2) {
+            const uint4 a = *reinterpret_cast<const uint4 *>(src + (size_t)k * N);
+            const uint4 b = *reinterpret_cast<const uint4 *>(src + (size_t)(k + 1) * N);
+            const uint32_t av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+            const int m = k >> 1;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {  // a spin is -1 iff its byte's sign bit is set
+                w0[j] |= (WT)((av[j >> 2] >> (8 * (j & 3) + 7)) & 1u) << m;
+                w1[j] |= (WT)((bv[j >> 2] >> (8 * (j & 3) + 7)) & 1u) << m;
+            }
+        }
+        WT *d0 = packed + (size_t)r * 2 * N + 16 * (size_t)q, *d1 = d0 + N;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            d0[j] = w0[j];
+            d1[j] = w1[j];
+        }
+    }
+}
+
+template <typename WT>
+__global__ void unpack_parity16_kernel(const WT *packed, int8_t *confs, int R, int P, int N)
+{
+    const int G = N / 16;
+    const size_t tot = (size_t)R * G;
+    for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < tot;
+         x += (size_t)gridDim.x * blockDim.x) {
+        const int q = (int)(x % G), r = (int)(x / G);
+        const WT *s0 = packed + (size_t)r * 2 * N + 16 * (size_t)q, *s1 = s0 + N;
+        WT w0[16], w1[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            w0[j] = s0[j];
+            w1[j] = s1[j];
+        }
+        int8_t *dst = confs + (size_t)r * P * N + 16 * (size_t)q;
+        for (int k = 0; k < P; k += 2) {
+            const int m = k >> 1;
+            uint32_t av[4] = {0u, 0u, 0u, 0u}, bv[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {  // bit set: -1 (0xFF), clear: +1 (0x01)
+                av[j >> 2] |= (1u | (uint32_t)((w0[j] >> m) & WT(1)) * 0xFEu) << (8 * (j & 3));
+                bv[j >> 2] |= (1u | (uint32_t)((w1[j] >> m) & WT(1)) * 0xFEu) << (8 * (j & 3));
+            }
+            *reinterpret_cast<uint4 *>(dst + (size_t)k * N) = make_uint4(av[0], av[1], av[2], av[3]);
+            *reinterpret_cast<uint4 *>(dst + (size_t)(k + 1) * N) = make_uint4(bv[0], bv[1], bv[2], bv[3]);
+        }
+    }
+}
+
 int grid_for(size_t n, int block)
 {
     size_t g = (n + block - 1) / block;
     return (int)(g > 65535 ? 65535 : (g < 1 ? 1 : g));
+}
+
+// int8 spins [R][P][N] <-> parity-packed words [R][2][N] on stream st.
+template <typename WT>
+void launch_pack(const int8_t *confs, WT *packed, int R, int P, int N, hipStream_t st)
+{
+    if (N % 16 == 0 && ((uintptr_t)confs & 15) == 0)
+        pack_parity16_kernel<WT><<<grid_for((size_t)R * (N / 16), 256), 256, 0, st>>>(confs, packed, R, P, N);
+    else
+        pack_parity_kernel<WT><<<grid_for((size_t)R * 2 * N, 256), 256, 0, st>>>(confs, packed, R, P, N);
+}
+
+template <typename WT>
+void launch_unpack(const WT *packed, int8_t *confs, int R, int P, int N, hipStream_t st)
+{
+    if (N % 16 == 0 && ((uintptr_t)confs & 15) == 0)
+        unpack_parity16_kernel<WT><<<grid_for((size_t)R * (N / 16), 256), 256, 0, st>>>(packed, confs, R, P, N);
+    else
+        unpack_parity_kernel<WT><<<grid_for((size_t)R * P * N, 256), 256, 0, st>>>(packed, confs, R, P, N);
 }
 
 // Per-field-step tables for the +-1 path.  The decision class of a bit is a
@@ -968,8 +1053,10 @@ bool pmj_tables(const piqmc_problem *pb, int S, const std::vector<float> &jp,
 {
     thr.assign((size_t)S * kClassStride, 0);
     ztab.assign((size_t)S * 8, 0);
+    // a uniform-degree lattice reads only its own degree's entries
+    const int zlo = pb->zuni >= 0 ? pb->zuni : 0, zhi = pb->zuni >= 0 ? pb->zuni : pb->zmax;
     for (int s = 0; s < S; ++s)
-        for (int z = 0; z <= pb->zmax; ++z)
+        for (int z = zlo; z <= zhi; ++z)
             for (int ut = 0; ut < 3; ++ut) {
                 int cls[9];
                 for (int us = 0; us <= z; ++us) {
@@ -1016,7 +1103,7 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         if (S) PIQMC_HIP(hipMemcpyAsync(d_jp, jp.data(), sizeof(float) * S, hipMemcpyHostToDevice, st));
         if (S) PIQMC_HIP(hipMemcpyAsync(d_jp + S, temps.data(), sizeof(float) * S, hipMemcpyHostToDevice, st));
         PIQMC_HIP(hipMallocAsync((void **)&d_packed, sizeof(WT) * 2 * (size_t)N * R, st));
-        pack_parity_kernel<WT><<<grid_for((size_t)R * 2 * N, 256), 256, 0, st>>>(confs, d_packed, R, P, N);
+        launch_pack<WT>(confs, d_packed, R, P, N, st);
         ColArgs<WT> a = {};
         a.N = N; a.Z = pb->Z; a.P = P; a.W = P / 2; a.S = S; a.M = M; a.R = R;
         a.ncolors = pb->ncolors; a.temps = d_jp + S;
@@ -1029,7 +1116,7 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         dense_kernel<WT><<<(R + wpb - 1) / wpb, block, lds, st>>>(a);
         piqmc::timing_end(st);
         PIQMC_HIP(hipGetLastError());
-        unpack_parity_kernel<WT><<<grid_for((size_t)R * P * N, 256), 256, 0, st>>>(d_packed, confs, R, P, N);
+        launch_unpack<WT>(d_packed, confs, R, P, N, st);
         PIQMC_HIP(hipFreeAsync(d_packed, st));
         PIQMC_HIP(hipFreeAsync(d_jp, st));
         PIQMC_HIP(hipGetLastError());
@@ -1060,7 +1147,7 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
     PIQMC_HIP(hipMallocAsync((void **)&d_coff, sizeof(int) * (pb->ncolors + 1), st));
     PIQMC_HIP(hipMemcpyAsync(d_coff, pb->color_off.data(), sizeof(int) * (pb->ncolors + 1), hipMemcpyHostToDevice, st));
     PIQMC_HIP(hipMallocAsync((void **)&d_packed, sizeof(WT) * 2 * (size_t)N * R, st));
-    pack_parity_kernel<WT><<<grid_for((size_t)R * 2 * N, 256), 256, 0, st>>>(confs, d_packed, R, P, N);
+    launch_pack<WT>(confs, d_packed, R, P, N, st);
 
     ColArgs<WT> a;
     a.N = N; a.Z = pb->Z; a.P = P; a.W = P / 2; a.S = S; a.M = M;
@@ -1102,7 +1189,7 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
     }
     piqmc::timing_end(st);
     PIQMC_HIP(hipGetLastError());
-    unpack_parity_kernel<WT><<<grid_for((size_t)R * P * N, 256), 256, 0, st>>>(d_packed, confs, R, P, N);
+    launch_unpack<WT>(d_packed, confs, R, P, N, st);
     PIQMC_HIP(hipFreeAsync(d_packed, st));
     PIQMC_HIP(hipFreeAsync(d_coff, st));
     if (d_thr) PIQMC_HIP(hipFreeAsync(d_thr, st));

@@ -288,3 +288,24 @@ def test_colored_stochastic_queue(gpu, monkeypatch, queue, L, dim, P, T, S, bloc
     assert pb.pm_j
     assert np.array_equal(got, ref)
     assert np.array_equal(etg, etr)
+
+
+@pytest.mark.parametrize("L,dim,P", [(8, 2, 64), (4, 2, 2), (4, 3, 128), (6, 2, 20), (12, 3, 66)])
+@pytest.mark.parametrize("offset", [0, 1])
+def test_colored_pack_roundtrip(gpu, L, dim, P, offset):
+    # a call without field steps packs the spins into parity words and back:
+    # the identity, through the 16-spin-group kernels (N % 16 == 0, aligned)
+    # and the per-element ones (other N, or spins one byte off alignment)
+    import torch
+    from piqmc import device
+    N, nbs, _, _, _ = pm_j_problem(L, dim=dim, seed=3)
+    pb = device.Problem(nbs)
+    R = 3
+    rng = np.random.RandomState(P + offset)
+    c0 = (2 * rng.randint(2, size=(R, P, N)) - 1).astype(np.int8)
+    base = torch.empty(R * P * N + 16, dtype=torch.int8, device=gpu)
+    c8 = base[offset:offset + R * P * N].view(R, P, N)
+    c8.copy_(torch.from_numpy(c0))
+    pb.qa_colored(c8, np.zeros(0), 1, 0.1, 1)
+    torch.cuda.synchronize()
+    assert np.array_equal(c8.cpu().numpy(), c0)
