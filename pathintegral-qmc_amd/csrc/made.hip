@@ -129,10 +129,14 @@ __global__ __launch_bounds__(256) void gemm_nt_bf16(const uint16_t *A, int lda, 
 }
 
 // Pipelined form for K, lda, ldb multiples of 64 / 8 / 8: K-stages of 64
-// (two MFMA k-steps), LDS double-buffered, the next stage's global loads in
-// flight while the current stage's MFMAs run, one barrier per stage.  Each
-// thread moves 2 x 16 B of A and of B per stage; rows past M / N load zeros.
+// (two MFMA k-steps) through a double-buffered LDS tile, with the global loads
+// of the next kNpf - 1 stages in flight in a register ring while the current
+// stage's MFMAs run (one workgroup per 64 x 64 tile leaves a CU one wavefront
+// per SIMD, so the stage loop itself has to hide the HBM latency), one barrier
+// per stage.  Each thread moves 2 x 16 B of A and of B per stage; rows past
+// M / N load row 0 and are zeroed when stored.
 constexpr int PK = 64, PKP = PK + 8;  // stage depth; LDS row pitch (bf16)
+constexpr int kNpf = 4;               // stages whose loads are issued ahead
 
 __global__ __launch_bounds__(256) void gemm_nt_bf16_pipe(const uint16_t *A, int lda, const uint16_t *B, int ldb,
                                                          int M, int N, int K, const float *bias, int relu,
@@ -155,46 +159,55 @@ __global__ __launch_bounds__(256) void gemm_nt_bf16_pipe(const uint16_t *A, int 
     const uint16_t *pa1 = A + (size_t)(a1 ? m0 + sr + 32 : 0) * lda + sc;
     const uint16_t *pb0 = B + (size_t)(b0 ? n0 + sr : 0) * ldb + sc;
     const uint16_t *pb1 = B + (size_t)(b1 ? n0 + sr + 32 : 0) * ldb + sc;
-    uint4 ra0, ra1, rb0, rb1;
-    auto fetch = [&](int k0) {  // rows past M / N read row 0 (in bounds); zeroed when stored
-        ra0 = *reinterpret_cast<const uint4 *>(pa0 + k0);
-        ra1 = *reinterpret_cast<const uint4 *>(pa1 + k0);
-        rb0 = *reinterpret_cast<const uint4 *>(pb0 + k0);
-        rb1 = *reinterpret_cast<const uint4 *>(pb1 + k0);
+    uint4 ra0[kNpf], ra1[kNpf], rb0[kNpf], rb1[kNpf];  // register ring: slot = stage % kNpf
+    auto fetch = [&](int slot, int stage) __attribute__((always_inline)) {
+        const int k0 = stage * PK;
+        ra0[slot] = *reinterpret_cast<const uint4 *>(pa0 + k0);
+        ra1[slot] = *reinterpret_cast<const uint4 *>(pa1 + k0);
+        rb0[slot] = *reinterpret_cast<const uint4 *>(pb0 + k0);
+        rb1[slot] = *reinterpret_cast<const uint4 *>(pb1 + k0);
     };
     const uint32_t ma0 = a0 ? ~0u : 0u, ma1 = a1 ? ~0u : 0u, mb0 = b0 ? ~0u : 0u, mb1 = b1 ? ~0u : 0u;
     auto masked = [](uint4 v, uint32_t m) { return make_uint4(v.x & m, v.y & m, v.z & m, v.w & m); };
-    auto stash = [&](int buf) {
-        *reinterpret_cast<uint4 *>(&As[buf][sr][sc]) = masked(ra0, ma0);
-        *reinterpret_cast<uint4 *>(&As[buf][sr + 32][sc]) = masked(ra1, ma1);
-        *reinterpret_cast<uint4 *>(&Bs[buf][sr][sc]) = masked(rb0, mb0);
-        *reinterpret_cast<uint4 *>(&Bs[buf][sr + 32][sc]) = masked(rb1, mb1);
+    auto stash = [&](int slot, int buf) __attribute__((always_inline)) {
+        *reinterpret_cast<uint4 *>(&As[buf][sr][sc]) = masked(ra0[slot], ma0);
+        *reinterpret_cast<uint4 *>(&As[buf][sr + 32][sc]) = masked(ra1[slot], ma1);
+        *reinterpret_cast<uint4 *>(&Bs[buf][sr][sc]) = masked(rb0[slot], mb0);
+        *reinterpret_cast<uint4 *>(&Bs[buf][sr + 32][sc]) = masked(rb1[slot], mb1);
     };
+    // Loads are issued unconditionally (stage indices past the end repeat the
+    // last stage), so the count of loads in flight is the same at every wait.
     const int ns = K / PK;
-    fetch(0);
-    stash(0);
+#pragma unroll
+    for (int j = 0; j < kNpf; ++j) fetch(j, min(j, ns - 1));
+    stash(0, 0);
     __syncthreads();
-    for (int s = 0; s < ns; ++s) {
-        const int buf = s & 1;
-        if (s + 1 < ns) fetch((s + 1) * PK);
+    for (int s0 = 0; s0 < ns; s0 += kNpf) {
 #pragma unroll
-        for (int kk = 0; kk < PK; kk += 32) {
-            bf16x8 af[2], bfr[2];
+        for (int j = 0; j < kNpf; ++j) {  // stage s = s0 + j sits in LDS buffer s & 1
+            const int s = s0 + j;
+            if (s >= ns) break;
+            const int buf = s & 1;
+            fetch(j, min(s + kNpf, ns - 1));  // slot j is free: stage s is already in LDS
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                uint4 va = *reinterpret_cast<const uint4 *>(&As[buf][wm + t * 16 + (lane & 15)][kk + (lane >> 4) * 8]);
-                uint4 vb = *reinterpret_cast<const uint4 *>(&Bs[buf][wn + t * 16 + (lane & 15)][kk + (lane >> 4) * 8]);
-                af[t] = __builtin_bit_cast(bf16x8, va);
-                bfr[t] = __builtin_bit_cast(bf16x8, vb);
+            for (int kk = 0; kk < PK; kk += 32) {
+                bf16x8 af[2], bfr[2];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    uint4 va = *reinterpret_cast<const uint4 *>(&As[buf][wm + t * 16 + (lane & 15)][kk + (lane >> 4) * 8]);
+                    uint4 vb = *reinterpret_cast<const uint4 *>(&Bs[buf][wn + t * 16 + (lane & 15)][kk + (lane >> 4) * 8]);
+                    af[t] = __builtin_bit_cast(bf16x8, va);
+                    bfr[t] = __builtin_bit_cast(bf16x8, vb);
+                }
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
             }
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+            stash((j + 1) % kNpf, buf ^ 1);  // after the last stage: unused
+            __syncthreads();
         }
-        if (s + 1 < ns) stash(buf ^ 1);
-        __syncthreads();
     }
     // accumulator layout: lane holds column (lane&15), rows (lane>>4)*4 + j
 #pragma unroll

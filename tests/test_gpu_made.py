@@ -18,7 +18,9 @@ def _bf16_bits(t):
     return t.to(torch.bfloat16).view(torch.int16)
 
 
-@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (70, 45, 100), (256, 512, 256), (1000, 300, 777)])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (70, 45, 100), (256, 512, 256), (1000, 300, 777),
+                                   (1024, 1024, 1024), (130, 70, 192), (96, 1728, 64), (2048, 512, 320),
+                                   (1024, 1024, 1728)])
 def test_gemm_bf16_matches_fp32_reference(gpu, M, N, K):
     import torch
     from piqmc import _lib
