@@ -66,6 +66,12 @@ int piqmc_problem_destroy(piqmc_problem *pb);
  * zmax: max couplings per spin. */
 int piqmc_problem_info(const piqmc_problem *pb, int *nspins, int *maxnb,
                        int *ncolors, int *pm_j, int *zmax);
+/* 1 if every row's local field sums exactly in double whatever the order
+ * (its couplings, as floats, are multiples of 2^L and the sum of their
+ * magnitudes is below 2^E with E - L <= 52), else 0.  The lane-per-slice
+ * sweep then splits each field over several lanes.  No counterpart in the
+ * reference: it reports the problem analysis the parallel sweep acts on. */
+int piqmc_problem_exact_rows(const piqmc_problem *pb, int *exact);
 /* Replace the spatial coloring (color_host[nspins], values 0..ncolors-1).
  * Rejected unless no two coupled spins share a color. */
 int piqmc_problem_set_coloring(piqmc_problem *pb, int ncolors, const int32_t *color_host);

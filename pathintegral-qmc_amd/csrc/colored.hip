@@ -754,14 +754,20 @@ __host__ __device__ inline size_t dense_wave_bytes(int N, int Z, int word_bytes)
     return state + (size_t)Z * 8 + (((size_t)Z * word_bytes + 7) & ~(size_t)7);
 }
 
-template <typename WT>
+template <typename WT, bool SPLIT = false>
 __device__ void dense_replica(const ColArgs<WT> &a, int r, unsigned char *base)
 {
     const int N = a.N, W = a.W, Z = a.Z, lane = threadIdx.x & 63;
     WT *st = reinterpret_cast<WT *>(base);
     double *jb = reinterpret_cast<double *>(base + (((size_t)2 * N * sizeof(WT) + 7) & ~(size_t)7));
     WT *wb = reinterpret_cast<WT *>(jb + Z);
-    const bool active = lane < W;
+    // With exactly summable rows a slice's field is split over `parts` lanes
+    // (slice lane % W, part lane / W) when W divides 16, and each lane keeps
+    // four partial sums: the same double, with more lanes busy and shorter
+    // dependency chains.  Lanes of part 0 decide.
+    const int parts = SPLIT ? 64 / W : 1;  // SPLIT: exact rows and W dividing 16
+    const int part = parts > 1 ? lane / W : 0;
+    const bool active = parts > 1 ? part == 0 : lane < W;
     const WT one = WT(1);
     const uint32_t rep = a.replica_base + (uint32_t)r;
     WT *gst = a.packed + (size_t)r * 2 * N;
@@ -806,7 +812,7 @@ __device__ void dense_replica(const ColArgs<WT> &a, int r, unsigned char *base)
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     if (q + 1 < N) fetch_row(__builtin_amdgcn_readfirstlane(a.order[q + 1]));
                     const WT S_ = mine[i], O = other[i];
-                    const int lb = lane < W ? lane : 0;
+                    const int lb = parts > 1 ? lane % W : (lane < W ? lane : 0);
                     // Trotter neighbours of slice 2lb+tau: slices 2lb+tau-1 and 2lb+tau+1
                     const int ml = tau == 0 ? (lb == 0 ? W - 1 : lb - 1) : lb;
                     const int mr = tau == 0 ? lb : (lb == W - 1 ? 0 : lb + 1);
@@ -824,7 +830,44 @@ __device__ void dense_replica(const ColArgs<WT> &a, int r, unsigned char *base)
                     double h = 0.0;
                     int e = 0;
                     constexpr int G = 8;  // entries per group; the next group is read ahead
-                    if (Z >= G) {
+                    if (SPLIT) {
+                        // any order gives the same double: this lane's entries are
+                        // part, part + parts, ...; groups of G read one group
+                        // ahead, four chains, then a butterfly over the parts
+                        double hk[4] = {0.0, 0.0, 0.0, 0.0};
+                        const int T = Z > part ? (Z - part + parts - 1) / parts : 0;
+                        int t = 0;
+                        if (T >= G) {
+                            double jc[G];
+                            WT wc[G];
+#pragma unroll
+                            for (int k = 0; k < G; ++k) {
+                                jc[k] = jb[part + parts * k];
+                                wc[k] = wb[part + parts * k];
+                            }
+                            for (; t + G <= T; t += G) {
+                                const int tn = min(t + G, T - G);
+                                double jn[G];
+                                WT wn[G];
+#pragma unroll
+                                for (int k = 0; k < G; ++k) {
+                                    jn[k] = jb[part + parts * (tn + k)];
+                                    wn[k] = wb[part + parts * (tn + k)];
+                                }
+#pragma unroll
+                                for (int k = 0; k < G; ++k) hk[k & 3] += term(jc[k], wc[k]);
+#pragma unroll
+                                for (int k = 0; k < G; ++k) {
+                                    jc[k] = jn[k];
+                                    wc[k] = wn[k];
+                                }
+                            }
+                        }
+                        for (; t < T; ++t) hk[t & 3] += term(jb[part + parts * t], wb[part + parts * t]);
+                        h = (hk[0] + hk[1]) + (hk[2] + hk[3]);
+                        for (int sh = W; sh < 64 && parts > 1; sh <<= 1) h += __shfl_xor(h, sh, 64);
+                        e = Z;
+                    } else if (Z >= G) {
                         double jc[G];
                         WT wc[G];
 #pragma unroll
@@ -879,7 +922,7 @@ __device__ void dense_replica(const ColArgs<WT> &a, int r, unsigned char *base)
                         }
                         if (active && cls == 2) fl = lt;
                     }
-                    const unsigned long long bal = __ballot(fl);
+                    const unsigned long long bal = __ballot(fl);  // part-0 lanes are slices 0..W-1
                     if (lane == 0) mine[i] = S_ ^ (WT)bal;
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
@@ -915,6 +958,19 @@ __global__ void dense_kernel(ColArgs<WT> a)
     const int w = threadIdx.x >> 6, r = blockIdx.x * (blockDim.x >> 6) + w;
     if (r >= a.R) return;
     dense_replica<WT>(a, r, smem + (size_t)w * dense_wave_bytes(a.N, a.Z, sizeof(WT)));
+}
+
+// The same with each slice's field split over 64 / W lanes (exactly summable
+// rows, W dividing 32); at most four wavefronts per workgroup, which leaves
+// the split sums their registers.
+constexpr int kDenseMaxBlock = 256;
+template <typename WT>
+__global__ void __launch_bounds__(kDenseMaxBlock) dense_split_kernel(ColArgs<WT> a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int w = threadIdx.x >> 6, r = blockIdx.x * (blockDim.x >> 6) + w;
+    if (r >= a.R) return;
+    dense_replica<WT, true>(a, r, smem + (size_t)w * dense_wave_bytes(a.N, a.Z, sizeof(WT)));
 }
 
 // int8 [R][P][N] -> packed [R][2][N] (bit m of parity tau = slice 2m+tau, set = -1)
@@ -1112,8 +1168,16 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         a.jperp = d_jp; a.rows = pb->d_rows; a.order = pb->d_order;
         a.packed = d_packed; a.etrace = etrace;
         piqmc::timing_begin(st);
-        (void)hipFuncSetAttribute((const void *)dense_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        dense_kernel<WT><<<(R + wpb - 1) / wpb, block, lds, st>>>(a);
+        const int W = P / 2;
+        // split when it gives at least four lanes per slice (P = 16: 2.2x,
+        // P = 32: 1.4x on 256-spin SK; two lanes per slice measured 4 % slower)
+        if (pb->exact_rows && W <= 16 && 16 % W == 0 && block <= kDenseMaxBlock) {
+            (void)hipFuncSetAttribute((const void *)dense_split_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            dense_split_kernel<WT><<<(R + wpb - 1) / wpb, block, lds, st>>>(a);
+        } else {
+            (void)hipFuncSetAttribute((const void *)dense_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            dense_kernel<WT><<<(R + wpb - 1) / wpb, block, lds, st>>>(a);
+        }
         piqmc::timing_end(st);
         PIQMC_HIP(hipGetLastError());
         launch_unpack<WT>(d_packed, confs, R, P, N, st);

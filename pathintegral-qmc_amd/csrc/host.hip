@@ -10,6 +10,8 @@
 #include <utility>
 #include <unordered_map>
 
+#include <climits>
+
 #include "piqmc_internal.h"
 
 namespace piqmc {
@@ -339,6 +341,23 @@ extern "C" int piqmc_problem_create(int N, int Z, const double *nbs, piqmc_probl
         }
         p->zmax = std::max(p->zmax, z);
     }
+    p->exact_rows = 1;
+    for (int i = 0; i < N && p->exact_rows; ++i) {
+        double bound = 0.0;
+        int lsb = INT_MAX;
+        for (int e = 0; e < Z; ++e) {
+            const float Jv = p->J[(size_t)i * Z + e];
+            if (Jv == 0.0f) continue;
+            bound += fabs((double)Jv);
+            int ex = 0;
+            (void)frexpf(Jv, &ex);                  // |Jv| = m 2^ex, 1/2 <= m < 1
+            lsb = std::min(lsb, std::max(ex - 24, -149));  // float: 24-bit significand
+        }
+        if (lsb == INT_MAX) continue;               // empty row
+        int eb = 0;
+        (void)frexp(bound, &eb);                    // every partial sum is below 2^eb
+        if (eb - lsb > 52) p->exact_rows = 0;
+    }
     if (p->zmax > 7) p->pmj = p->pmjf = 0;  // bit-sliced counter holds 0..7
     if (p->pmj || nfield == 0) p->pmjf = 0;
     p->zuni = p->zmax;
@@ -437,6 +456,13 @@ extern "C" int piqmc_problem_info(const piqmc_problem *p, int *n, int *z, int *n
     if (nc) *nc = p->ncolors;
     if (pmj) *pmj = p->pmj ? 1 : (p->pmjf ? 2 : 0);
     if (zmax) *zmax = p->zmax;
+    return PIQMC_OK;
+}
+
+extern "C" int piqmc_problem_exact_rows(const piqmc_problem *p, int *exact)
+{
+    if (!p || !exact) return set_error(PIQMC_EINVAL, "null argument");
+    *exact = p->exact_rows;
     return PIQMC_OK;
 }
 

@@ -55,6 +55,11 @@ struct piqmc_problem {
     // 2^-27 <= |h| <= 2^20 (so the local field's partial sums are exact)
     int pmjf = 0;
     float *d_hq = nullptr;        // [N] field of spin order[q]
+    // Every row's local field sums exactly in double, in any order (all of
+    // its couplings are multiples of 2^L and all partial sums stay below 2^E
+    // with E - L <= 52): the lane-per-slice sweep may then split and
+    // reassociate the sum without changing a bit.
+    int exact_rows = 0;
 };
 
 // ------------------------------------------------------------------ device helpers

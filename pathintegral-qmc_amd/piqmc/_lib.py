@@ -29,6 +29,7 @@ _SIGS = {
     "piqmc_problem_create_dense": ([_i, _vp, _i64, _i64, C.POINTER(_vp)], _i),
     "piqmc_problem_destroy": ([_vp], _i),
     "piqmc_problem_info": ([_vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "piqmc_problem_exact_rows": ([_vp, _vp], _i),
     "piqmc_problem_set_coloring": ([_vp, _i, _vp], _i),
     "piqmc_problem_get_coloring": ([_vp, _vp], _i),
     "piqmc_problem_set_sweep_path": ([_vp, _i], _i),

@@ -106,6 +106,9 @@ class Problem:
         self.ncolors, self.zmax = vals[2].value, vals[4].value
         self.pm_j = vals[3].value == 1            # +-1 couplings only
         self.pm_j_fields = vals[3].value == 2     # +-1 couplings and fields
+        ex = C.c_int()
+        _lib.call("piqmc_problem_exact_rows", self._h, C.byref(ex))
+        self.exact_rows = bool(ex.value)          # local fields sum exactly in any order
 
     def close(self):
         if getattr(self, "_h", None):

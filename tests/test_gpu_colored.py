@@ -168,16 +168,23 @@ def test_colored_statistics_vs_sequential_oracle(gpu):
     assert abs(pg - ps) <= 3 * np.sqrt(max(pp * (1 - pp), 1e-12) * 2 / nseed) + 1e-12, (pg, ps)
 
 
-@pytest.mark.parametrize("N,P", [(64, 16), (100, 32), (256, 128), (40, 20)])
-def test_dense_path_sk_bit_exact(gpu, N, P):
+@pytest.mark.parametrize("tiny", [False, True])
+@pytest.mark.parametrize("N,P", [(64, 16), (100, 32), (256, 128), (40, 20), (48, 8), (64, 2), (80, 64)])
+def test_dense_path_sk_bit_exact(gpu, N, P, tiny):
     # Sherrington-Kirkpatrick couplings J_ij ~ N(0, 1/N): one color per spin,
-    # the lane-per-slice path, against the CPU twin
+    # the lane-per-slice path, against the CPU twin.  Such rows sum exactly in
+    # double, so each field is split over 64/(P/2) lanes when P/2 divides 64;
+    # one coupling of 1e-12 makes a row inexact and keeps the row-order sum.
     from piqmc import instances
     n, r, c, v = instances.sk(N, np.random.RandomState(N))
+    if tiny:
+        v = v.copy()
+        v[7] = 1e-12
     nbs = instances.neighbors(n, r, c, v)
     pb, got, ref, etg, etr = _run(gpu, nbs, P, np.linspace(3.0, 1e-8, 6), 1, 0.05, R=5, seed=N + P,
                                   rbase=3)
     assert pb.ncolors == N
+    assert pb.exact_rows == (not tiny)
     assert np.array_equal(got, ref)
     np.testing.assert_allclose(etg, etr, rtol=1e-12, atol=1e-10)
 
