@@ -714,6 +714,14 @@ inline bool queue_enabled()
     return !(e && e[0] == '0');
 }
 
+// Exactly summable dense problems sweep one replica per workgroup unless
+// PIQMC_DENSE_WG=0 (identical results; the switch exists to compare).
+inline bool dense_wg_enabled()
+{
+    const char *e = std::getenv("PIQMC_DENSE_WG");
+    return !(e && e[0] == '0');
+}
+
 // LDS of the word-per-thread sweep with the stochastic queues of block/64 wavefronts.
 inline size_t queue_lds(int N, int rpw, int word_bytes, int block)
 {
@@ -950,6 +958,150 @@ __device__ void dense_replica(const ColArgs<WT> &a, int r, unsigned char *base)
     for (int q = lane; q < 2 * N; q += 64) gst[q] = st[q];
 }
 
+// LDS of the workgroup-per-replica dense sweep: state [2][N], staged row
+// (couplings [Z] as double, neighbour words [Z]), partial fields [4][64].
+__host__ __device__ inline size_t dense_wg_bytes(int N, int Z, int word_bytes)
+{
+    return dense_wave_bytes(N, Z, word_bytes) + 4 * 64 * sizeof(double);
+}
+
+// Dense couplings whose rows sum exactly in double in any order, one replica
+// per workgroup of four wavefronts.  Per spin the whole workgroup stages the
+// row, every wavefront sums a quarter of it for all slices (lane = slice, and
+// 64/W lanes per slice when W divides 32), and wavefront 0 adds the four
+// partial fields and decides exactly as the lane-per-slice body does: the
+// same doubles, the same Philox words, the same flips.  Three barriers per
+// spin buy four times the lanes for one replica's dependency chain.
+template <typename WT>
+__device__ void dense_wg_replica(const ColArgs<WT> &a, int r, unsigned char *base)
+{
+    const int N = a.N, W = a.W, Z = a.Z, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    WT *st = reinterpret_cast<WT *>(base);
+    double *jb = reinterpret_cast<double *>(base + (((size_t)2 * N * sizeof(WT) + 7) & ~(size_t)7));
+    WT *wb = reinterpret_cast<WT *>(jb + Z);
+    double *ps = reinterpret_cast<double *>(base + dense_wave_bytes(N, Z, sizeof(WT)));  // [4][64]
+    const int parts = (W <= 32 && 64 % W == 0) ? 64 / W : 1;
+    const int lb = parts > 1 ? lane % W : (lane < W ? lane : 0);
+    const int gp = wv * parts + (parts > 1 ? lane / W : 0), stride = 4 * parts;
+    const bool active = lane < W;  // in wavefront 0: the lanes that decide slice lb
+    const WT one = WT(1);
+    const uint32_t rep = a.replica_base + (uint32_t)r;
+    WT *gst = a.packed + (size_t)r * 2 * N;
+    for (int q = tid; q < 2 * N; q += 256) st[q] = gst[q];
+    constexpr int kPre = 2;  // row entries tid + 256 k, read one spin ahead
+    int2 pre[kPre];
+    auto fetch_row = [&](int i) {
+        const int2 *row = a.rows + (size_t)i * Z;
+#pragma unroll
+        for (int k = 0; k < kPre; ++k) pre[k] = row[min(tid + 256 * k, Z - 1)];
+    };
+    const uint32_t shl = 31u - (uint32_t)(lb & 31);
+    auto term = [&](double J, WT w) __attribute__((always_inline)) {
+        const uint32_t half = sizeof(WT) == 8 && lb >= 32 ? (uint32_t)((uint64_t)w >> 32) : (uint32_t)w;
+        return __longlong_as_double(__double_as_longlong(J) ^ ((long long)((half << shl) & 0x80000000u) << 32));
+    };
+    for (int s = 0; s < a.S; ++s) {
+        const float jp = a.jperp[s];
+        const double Ts = (double)a.temps[s];
+        for (int m = 0; m < a.M; ++m) {
+            const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
+            for (int tau = 0; tau < 2; ++tau) {
+                WT *mine = st + tau * N;
+                const WT *other = st + (1 - tau) * N;
+                fetch_row(__builtin_amdgcn_readfirstlane(a.order[0]));
+                for (int q = 0; q < N; ++q) {
+                    const int i = __builtin_amdgcn_readfirstlane(a.order[q]);
+#pragma unroll
+                    for (int k = 0; k < kPre; ++k) {
+                        const int e = tid + 256 * k;
+                        if (e < Z) {
+                            jb[e] = (double)__int_as_float(pre[k].y);
+                            wb[e] = pre[k].x == i ? WT(0) : mine[pre[k].x];
+                        }
+                    }
+                    for (int e = tid + 256 * kPre; e < Z; e += 256) {
+                        const int2 rw = a.rows[(size_t)i * Z + e];
+                        jb[e] = (double)__int_as_float(rw.y);
+                        wb[e] = rw.x == i ? WT(0) : mine[rw.x];
+                    }
+                    __syncthreads();
+                    if (q + 1 < N) fetch_row(__builtin_amdgcn_readfirstlane(a.order[q + 1]));
+                    double h0 = 0.0, h1 = 0.0;
+                    int e = gp;
+                    for (; e + stride < Z; e += 2 * stride) {
+                        h0 += term(jb[e], wb[e]);
+                        h1 += term(jb[e + stride], wb[e + stride]);
+                    }
+                    if (e < Z) h0 += term(jb[e], wb[e]);
+                    double hp = h0 + h1;
+                    for (int sh = W; sh < 64 && parts > 1; sh <<= 1) hp += __shfl_xor(hp, sh, 64);
+                    if (lane < W) ps[wv * 64 + lane] = hp;
+                    __syncthreads();
+                    if (wv == 0) {
+                        const double h = (ps[lb] + ps[64 + lb]) + (ps[128 + lb] + ps[192 + lb]);
+                        const WT S_ = mine[i], O = other[i];
+                        const int ml = tau == 0 ? (lb == 0 ? W - 1 : lb - 1) : lb;
+                        const int mr = tau == 0 ? lb : (lb == W - 1 ? 0 : lb + 1);
+                        const double si = ((S_ >> lb) & one) ? -1.0 : 1.0;
+                        const double ts = (((O >> ml) & one) ? -1.0 : 1.0) + (((O >> mr) & one) ? -1.0 : 1.0);
+                        const double ed = -2.0 * si * (h + (double)jp * ts);
+                        unsigned int thr = 0;
+                        const int cls = piqmc_fixed_class(ed, Ts, &thr);
+                        bool fl = active && cls == 1;
+                        if (__ballot(active && cls == 2)) {
+                            constexpr int kPer = sizeof(WT) == 4 ? 4 : 2;
+                            bool und = active && cls == 2, lt = false;
+                            for (int qd = 0; qd < 32 / kPer && __ballot(und); ++qd) {
+                                uint32_t out[4];
+                                philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd, a.key0,
+                                              a.key1, out);
+#pragma unroll
+                                for (int t = 0; t < kPer; ++t) {
+                                    const uint32_t ub = (uint32_t)((rword<WT>(out, t) >> lb) & one);
+                                    const uint32_t tb = (thr >> (31 - (qd * kPer + t))) & 1u;
+                                    if (und && ub != tb) {
+                                        lt = ub < tb;
+                                        und = false;
+                                    }
+                                }
+                            }
+                            if (active && cls == 2) fl = lt;
+                        }
+                        const unsigned long long bal = __ballot(fl);
+                        if (lane == 0) mine[i] = S_ ^ (WT)bal;
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+        if (a.etrace && wv == 0) {
+            double acc = 0.0;
+            for (int i = lane; i < N; i += 64) {
+                const WT a0 = st[i], a1 = st[N + i];
+                const int2 *row = a.rows + (size_t)i * a.Z;
+                for (int e = 0; e < a.Z; ++e) {
+                    const int2 rw = row[e];
+                    const double J = (double)__int_as_float(rw.y);
+                    if (J == 0.0) continue;
+                    if (rw.x == i) acc -= J * (double)(a.P - 2 * (popc(a0) + popc(a1)));
+                    else acc -= 0.5 * J * (double)(a.P - 2 * (popc(a0 ^ st[rw.x]) + popc(a1 ^ st[N + rw.x])));
+                }
+            }
+            acc = wave_sum(acc);
+            if (lane == 0) a.etrace[(size_t)r * a.S + s] = acc / (double)a.P;
+        }
+    }
+    __syncthreads();
+    for (int q = tid; q < 2 * N; q += 256) gst[q] = st[q];
+}
+
+template <typename WT>
+__global__ void __launch_bounds__(256) dense_wg_kernel(ColArgs<WT> a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    dense_wg_replica<WT>(a, blockIdx.x, smem);
+}
+
 // One replica per wavefront, four wavefronts per workgroup, no barriers.
 template <typename WT>
 __global__ void dense_kernel(ColArgs<WT> a)
@@ -1152,7 +1304,11 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         if (block > 1024 || (block & 63)) return piqmc::set_error(PIQMC_EINVAL, "block must be a multiple of 64, <= 1024");
         const int wpb = block / 64;
         const size_t lds = dense_wave_bytes(N, pb->Z, sizeof(WT)) * wpb;
-        if (lds > 160 * 1024) return piqmc::set_error(PIQMC_EINVAL, "dense sweep: state exceeds LDS");
+        const size_t lwg = dense_wg_bytes(N, pb->Z, sizeof(WT));
+        // a workgroup per replica while replicas are few (1.2-1.9x up to 1024
+        // 256-spin replicas); from 4096 on, a wavefront each is 1.3x faster
+        const bool wg = pb->exact_rows && dense_wg_enabled() && lwg <= 160 * 1024 && R < 2048;
+        if (!wg && lds > 160 * 1024) return piqmc::set_error(PIQMC_EINVAL, "dense sweep: state exceeds LDS");
         float *d_jp = nullptr;  // [S] J_perp, then [S] temperatures
         WT *d_packed = nullptr;
         PIQMC_HIP(hipMallocAsync((void **)&d_jp, sizeof(float) * (S ? 2 * S : 1), st));
@@ -1169,9 +1325,13 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         a.packed = d_packed; a.etrace = etrace;
         piqmc::timing_begin(st);
         const int W = P / 2;
-        // split when it gives at least four lanes per slice (P = 16: 2.2x,
-        // P = 32: 1.4x on 256-spin SK; two lanes per slice measured 4 % slower)
-        if (pb->exact_rows && W <= 16 && 16 % W == 0 && block <= kDenseMaxBlock) {
+        if (wg) {
+            // one replica per workgroup (the field split over four wavefronts)
+            (void)hipFuncSetAttribute((const void *)dense_wg_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lwg);
+            dense_wg_kernel<WT><<<R, 256, lwg, st>>>(a);
+        } else if (pb->exact_rows && W <= 16 && 16 % W == 0 && block <= kDenseMaxBlock) {
+            // split when it gives at least four lanes per slice (P = 16: 2.2x,
+            // P = 32: 1.4x on 256-spin SK; two lanes per slice measured 4 % slower)
             (void)hipFuncSetAttribute((const void *)dense_split_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             dense_split_kernel<WT><<<(R + wpb - 1) / wpb, block, lds, st>>>(a);
         } else {

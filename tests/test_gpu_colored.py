@@ -168,13 +168,17 @@ def test_colored_statistics_vs_sequential_oracle(gpu):
     assert abs(pg - ps) <= 3 * np.sqrt(max(pp * (1 - pp), 1e-12) * 2 / nseed) + 1e-12, (pg, ps)
 
 
+@pytest.mark.parametrize("wg", ["1", "0"])
 @pytest.mark.parametrize("tiny", [False, True])
 @pytest.mark.parametrize("N,P", [(64, 16), (100, 32), (256, 128), (40, 20), (48, 8), (64, 2), (80, 64)])
-def test_dense_path_sk_bit_exact(gpu, N, P, tiny):
+def test_dense_path_sk_bit_exact(gpu, monkeypatch, N, P, tiny, wg):
     # Sherrington-Kirkpatrick couplings J_ij ~ N(0, 1/N): one color per spin,
-    # the lane-per-slice path, against the CPU twin.  Such rows sum exactly in
-    # double, so each field is split over 64/(P/2) lanes when P/2 divides 64;
-    # one coupling of 1e-12 makes a row inexact and keeps the row-order sum.
+    # the dense paths, against the CPU twin.  Such rows sum exactly in double:
+    # one replica per workgroup with the field split over four wavefronts, or
+    # (PIQMC_DENSE_WG=0) one replica per wavefront with the field split over
+    # 64/(P/2) lanes when P/2 divides 16.  One coupling of 1e-12 makes a row
+    # inexact and keeps the row-order sum of one lane per slice.
+    monkeypatch.setenv("PIQMC_DENSE_WG", wg)
     from piqmc import instances
     n, r, c, v = instances.sk(N, np.random.RandomState(N))
     if tiny:
