@@ -307,8 +307,10 @@ int run_replay(const piqmc_problem *pb, int R, int P, int S, int M, float temp, 
     const int nreg = sa ? 1 : (P == 2 ? 2 : 3);
     // Launch shape: W waves per workgroup share one LDS copy of the neighbour
     // rows; Lw lanes per wave each own a replica.  Lw is the smallest power of
-    // two that still gives every CU a workgroup, so replicas spread over all
-    // SIMDs.  The staged form (rows and the sweep's permutations in LDS) is
+    // two that still gives every CU two workgroups, so replicas spread over all
+    // SIMDs with two chains' latency to hide per SIMD (4096 replicas of the
+    // 1024-spin lattice: 8.4e9 attempts/s at 2 lanes per wave, 5.2e9 at 4, 3.8e9
+    // at 1).  The staged form (rows and the sweep's permutations in LDS) is
     // preferred; shapes shrink until it fits, else rows and permutations are
     // read from global memory.
     const int Zp = (pb->Z + 3) & ~3;
@@ -319,7 +321,7 @@ int run_replay(const piqmc_problem *pb, int R, int P, int S, int M, float temp, 
     constexpr size_t kLds = 160 * 1024;
     constexpr int kCUs = 256;
     int W0 = 4, Lw0 = 64;
-    while (Lw0 > 1 && (size_t)W0 * Lw0 * kCUs > (size_t)R) Lw0 /= 2;
+    while (Lw0 > 1 && (size_t)W0 * Lw0 * kCUs * 2 > (size_t)R) Lw0 /= 2;
     int W = W0, Lw = Lw0;
     bool staged = true;
     while (lds_of(W * Lw, true) > kLds && (W > 1 || Lw > 1)) {
