@@ -871,7 +871,7 @@ __device__ void dense_replica(const ColArgs<WT> &a, int r, unsigned char *base)
                                 }
                             }
                         }
-                        for (; t < T; ++t) hk[t & 3] += term(jb[part + parts * t], wb[part + parts * t]);
+                        for (; t < T; ++t) hk[0] += term(jb[part + parts * t], wb[part + parts * t]);
                         h = (hk[0] + hk[1]) + (hk[2] + hk[3]);
                         for (int sh = W; sh < 64 && parts > 1; sh <<= 1) h += __shfl_xor(h, sh, 64);
                         e = Z;
@@ -1026,14 +1026,50 @@ __device__ void dense_wg_replica(const ColArgs<WT> &a, int r, unsigned char *bas
                     }
                     __syncthreads();
                     if (q + 1 < N) fetch_row(__builtin_amdgcn_readfirstlane(a.order[q + 1]));
-                    double h0 = 0.0, h1 = 0.0;
-                    int e = gp;
-                    for (; e + stride < Z; e += 2 * stride) {
-                        h0 += term(jb[e], wb[e]);
-                        h1 += term(jb[e + stride], wb[e + stride]);
+                    // this lane's entries gp, gp + stride, ...: with one lane per
+                    // slice (long runs) in groups of G read one group ahead, four
+                    // chains; with several lanes per slice (runs of 8-32) two
+                    // chains read in place were measured faster
+                    constexpr int G = 8;
+                    double hk[4] = {0.0, 0.0, 0.0, 0.0};
+                    const int T = Z > gp ? (Z - gp + stride - 1) / stride : 0;
+                    int t = 0;
+                    if (parts > 1) {
+                        int e = gp;
+                        for (; e + stride < Z; e += 2 * stride) {
+                            hk[0] += term(jb[e], wb[e]);
+                            hk[1] += term(jb[e + stride], wb[e + stride]);
+                        }
+                        if (e < Z) hk[0] += term(jb[e], wb[e]);
+                        t = T;
+                    } else if (T >= G) {
+                        double jc[G];
+                        WT wc[G];
+#pragma unroll
+                        for (int k = 0; k < G; ++k) {
+                            jc[k] = jb[gp + stride * k];
+                            wc[k] = wb[gp + stride * k];
+                        }
+                        for (; t + G <= T; t += G) {
+                            const int tn = min(t + G, T - G);
+                            double jn[G];
+                            WT wn[G];
+#pragma unroll
+                            for (int k = 0; k < G; ++k) {
+                                jn[k] = jb[gp + stride * (tn + k)];
+                                wn[k] = wb[gp + stride * (tn + k)];
+                            }
+#pragma unroll
+                            for (int k = 0; k < G; ++k) hk[k & 3] += term(jc[k], wc[k]);
+#pragma unroll
+                            for (int k = 0; k < G; ++k) {
+                                jc[k] = jn[k];
+                                wc[k] = wn[k];
+                            }
+                        }
                     }
-                    if (e < Z) h0 += term(jb[e], wb[e]);
-                    double hp = h0 + h1;
+                    for (; t < T; ++t) hk[0] += term(jb[gp + stride * t], wb[gp + stride * t]);
+                    double hp = (hk[0] + hk[1]) + (hk[2] + hk[3]);
                     for (int sh = W; sh < 64 && parts > 1; sh <<= 1) hp += __shfl_xor(hp, sh, 64);
                     if (lane < W) ps[wv * 64 + lane] = hp;
                     __syncthreads();
