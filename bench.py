@@ -30,20 +30,36 @@ COUNTERS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         "colored_pmj1024_counters.md")
 
 
-def measured_traffic(R, N, P, S):
-    """HBM bytes per launch of the headline kernel, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a 1 GiB
-    device copy in the same run), when it covers this workload; else None."""
+ISSUE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "colored_pmj1024_issue.md")
+
+
+def _profile_figure(path, tag, R, N, P, S):
+    """A figure `tag: value R=.. N=.. P=.. S=..` from a committed rocprofv3
+    summary, when it covers this workload; else None."""
     try:
-        with open(COUNTERS) as f:
+        with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(tag + ":"):
                     _, val, key = line.split(None, 2)
                     if key.strip() == "R=%d N=%d P=%d S=%d" % (R, N, P, S):
                         return float(val)
     except OSError:
         pass
     return None
+
+
+def measured_traffic(R, N, P, S):
+    """HBM bytes per launch of the headline kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a 1 GiB
+    device copy in the same run), when it covers this workload; else None."""
+    return _profile_figure(COUNTERS, "traffic_bytes_per_launch", R, N, P, S)
+
+
+def measured_valu_issue(R, N, P, S):
+    """Fraction of the SIMDs' cycles spent issuing VALU work in the headline
+    kernel (SQ_INSTS_VALU x 4 / (1024 SIMDs x GRBM_GUI_ACTIVE per XCD)), from
+    the committed rocprofv3 summary; else None."""
+    return _profile_figure(ISSUE, "valu_issue_fraction", R, N, P, S)
 
 
 def cpu_baseline(nbs, N, P, T, sched, reps):
@@ -274,7 +290,8 @@ def main():
                      "kernel": "colored_kernel<uint32,+-1 path>", "tile": {"block": tile[0], "rpw": tile[1]},
                      "kernel_ms": kavg * 1e3, "kernel_attempts_per_s": k_rate,
                      "bytes_per_attempt": BYTES_PER_ATTEMPT,
-                     "lds_frac": achieved / LDS_PEAK_GBS},
+                     "lds_frac": achieved / LDS_PEAK_GBS,
+                     "valu_issue_frac": measured_valu_issue(R, N, P, S)},
         "mean_min_slice_energy_per_spin": e_min,
     }
 
