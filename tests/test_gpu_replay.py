@@ -86,6 +86,14 @@ def test_qa_replay_many_replicas(gpu):
                 seeds=list(range(130)), rseeds=list(range(1000, 1130)))
 
 
+def test_qa_replay_wide_launch(gpu):
+    # 4100 replicas: the launch shape of the 4096-replica workload (two lanes
+    # per wave, two workgroups per CU) plus a partial last workgroup
+    N, nbs, _, _, _ = pm_j_problem(4, seed=3)
+    _run_replay(gpu, nbs, 4, np.linspace(3.0, 1e-8, 5), 1, 0.25,
+                seeds=list(range(4100)), rseeds=list(range(5000, 9100)))
+
+
 def test_qa_replay_full_size_subset(gpu):
     # 1024-spin +-J glass, P=64, T=1/64, 500 field steps (2 replicas vs oracle)
     N, nbs, _, _, _ = pm_j_problem(32, seed=2024)
