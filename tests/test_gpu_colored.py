@@ -114,6 +114,28 @@ def test_colored_tile_invariance(gpu, field_sigma):
     assert all(np.array_equal(outs[0], o) for o in outs)
 
 
+def test_colored_full_size_launch_vs_oracle(gpu):
+    # the measured configuration itself: 1024-spin glass, P=64, T=1/64, 500
+    # field steps, 4096 replicas in one launch with the default tiles (queue,
+    # cached neighbours); the first and last replica against the CPU twin
+    import torch
+    from piqmc import device
+    N, nbs, _, _, _ = pm_j_problem(32, seed=2024)
+    pb = device.Problem(nbs)
+    R, P, T = 4096, 64, 1.0 / 64
+    sched = np.linspace(3.0, 1e-8, 500)
+    g = torch.Generator(device="cpu").manual_seed(0)
+    c0 = (torch.randint(0, 2, (R, P, N), generator=g, dtype=torch.int8) * 2 - 1)
+    c8 = c0.to(gpu)
+    pb.qa_colored(c8, sched, 1, T, 12345)
+    torch.cuda.synchronize()
+    color = pb.coloring()
+    for r in (0, R - 1):
+        sp = c0[r].numpy().copy()
+        O.qa_colored(sched, 1, P, T, sp, nbs, color, int(color.max()) + 1, 12345, replica=r)
+        assert np.array_equal(c8[r].cpu().numpy(), sp), "replica %d" % r
+
+
 def test_colored_full_size_properties(gpu):
     # 1024-spin glass, P=64, T=1/64, 500 steps, 256 replicas: per-replica energy
     # trace consistent with a final slice-energy evaluation, spins stay +-1,
