@@ -273,379 +273,390 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
         const float jp = a.jperp[s];
         const double Ts = (double)a.temps[s];
         const bool cq = CQ && un <= 3;  // this step's stochastic words go through the queue
-        for (int m = 0; m < a.M; ++m) {
-            const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
-            for (int tau = 0; tau < 2; ++tau) {
-                const int tsa = tau == 0 ? 1 : W - 1, tsb = tau == 0 ? W - 1 : 1;  // rotation by one slice
-                // The top cnt queued words, one per lane: each word's stochastic
-                // bits meet the same bit-sliced uniforms, thresholds and early
-                // exit as in the inline path, and the accepted ones are XORed
-                // into the word (its deterministic flips are already stored).
-                // NS: classes of the step (1, or up to 3), a compile-time constant.
-                auto drain = [&](const int cnt, auto NS) __attribute__((always_inline)) {
-                    __builtin_amdgcn_wave_barrier();
-                    const int lane = tid & 63;
-                    if (lane < cnt) {
-                        uint4 rec;  // single-class records are (index | replica << 16, mask)
-                        if constexpr (NS() == 1) {
-                            const uint2 r2 = *reinterpret_cast<const uint2 *>(wq + (qn - cnt + lane));
-                            rec = make_uint4(r2.x, r2.y, 0u, 0u);
-                        } else {
-                            rec = wq[qn - cnt + lane];
-                        }
-                        const int i = (int)(rec.x & 0xffffu), rr = (int)(rec.x >> 16);
-                        const WT m0 = (WT)rec.y, m1 = NS() > 1 ? (WT)rec.z : WT(0), m2 = NS() > 2 ? (WT)rec.w : WT(0);
-                        const uint32_t rep = a.replica_base + (uint32_t)(r0 + rr);
-                        WT lt = 0, und = m0 | m1 | m2;
-                        for (int qd = 0; qd < 8 && und; ++qd) {
-                            uint32_t out[4];
-                            philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd, a.key0, a.key1, out);
-#pragma unroll
-                            for (int t = 0; t < 4; ++t) {
-                                const WT Rb = rword<WT>(out, t);
-                                const int sh = 31 - (qd * 4 + t);
-                                const WT tb = (((th_s[0] >> sh) & 1u) ? m0 : WT(0)) |
-                                              (((th_s[1] >> sh) & 1u) ? m1 : WT(0)) |
-                                              (((th_s[2] >> sh) & 1u) ? m2 : WT(0));
-                                lt |= und & ~Rb & tb;
-                                und &= ~(Rb ^ tb);
+        // Steps without stochastic classes (CQ kernels) run a copy of the
+        // sweep with every stochastic branch compiled out.
+        auto sweeps = [&](auto DETc) __attribute__((always_inline)) {
+            constexpr bool DET = decltype(DETc)::value;
+            for (int m = 0; m < a.M; ++m) {
+                const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
+                for (int tau = 0; tau < 2; ++tau) {
+                    const int tsa = tau == 0 ? 1 : W - 1, tsb = tau == 0 ? W - 1 : 1;  // rotation by one slice
+                    // The top cnt queued words, one per lane: each word's stochastic
+                    // bits meet the same bit-sliced uniforms, thresholds and early
+                    // exit as in the inline path, and the accepted ones are XORed
+                    // into the word (its deterministic flips are already stored).
+                    // NS: classes of the step (1, or up to 3), a compile-time constant.
+                    auto drain = [&](const int cnt, auto NS) __attribute__((always_inline)) {
+                        __builtin_amdgcn_wave_barrier();
+                        const int lane = tid & 63;
+                        if (lane < cnt) {
+                            uint4 rec;  // single-class records are (index | replica << 16, mask)
+                            if constexpr (NS() == 1) {
+                                const uint2 r2 = *reinterpret_cast<const uint2 *>(wq + (qn - cnt + lane));
+                                rec = make_uint4(r2.x, r2.y, 0u, 0u);
+                            } else {
+                                rec = wq[qn - cnt + lane];
                             }
-                        }
-                        st[((size_t)rr * 2 + tau) * N + i] ^= lt;
-                    }
-                    qn -= cnt;
-                    __builtin_amdgcn_wave_barrier();
-                };
-                // one spin (position q in color order, index i, neighbours nb)
-                // for every replica of the workgroup; lanes with act false take
-                // part only in the queue's wave-wide steps
-                auto spin = [&](const bool act, const int q, const int i, const uint32_t info, const int zi,
-                                const int *nb) __attribute__((always_inline)) {
-                        const uint32_t zt = UZ ? zt_u : ((PMJ && !FLD) ? tz[zi] : 0u);
-                        // fields: this spin's class thresholds, shared by the replicas below
-                        double hfield = 0.0;
-                        FieldClasses fc;
-                        if (FLD) {
-                            hfield = (double)a.hq[q];
-                            field_classes(hfield, zi, jp, Ts, fc);
-                        }
-                        for (int rr = 0; rr < nrep; ++rr) {
-                          WT qX = 0, qm0 = 0, qm1 = 0, qm2 = 0;  // stochastic word for the queue
-                          if (act) {
-                            WT *mine = st + ((size_t)rr * 2 + tau) * N;
-                            const WT *other = st + ((size_t)rr * 2 + 1 - tau) * N;
+                            const int i = (int)(rec.x & 0xffffu), rr = (int)(rec.x >> 16);
+                            const WT m0 = (WT)rec.y, m1 = NS() > 1 ? (WT)rec.z : WT(0), m2 = NS() > 2 ? (WT)rec.w : WT(0);
                             const uint32_t rep = a.replica_base + (uint32_t)(r0 + rr);
-                            const WT S_ = mine[i], O = other[i];
-                            // Trotter neighbours of slice 2m+tau: bits m of O and of O
-                            // rotated by one slice (left for tau 0, right for tau 1);
-                            // everything below is symmetric in the two
-                            const WT Lw = O, Rw = ((O << tsa) | (O >> tsb)) & valid;
-                            const WT uL = S_ ^ Lw, uR = S_ ^ Rw;  // 1 = antiparallel Trotter bond
-                            WT flips = 0;
-                            if (PMJ) {
-                                WT c0 = 0, c1 = 0, c2 = 0;  // bit-sliced unsatisfied-bond counter
-#pragma unroll
-                                for (int z = 0; z < 8; ++z) {
-                                    if (ZU > 0 ? z < ZU : (z < a.zmax && z < zi)) {
-                                        const WT x = S_ ^ mine[nb[z]] ^ (((info >> z) & 1u) ? ~WT(0) : WT(0));
-                                        const WT k0 = c0 & x;
-                                        c0 ^= x;
-                                        const WT k1 = c1 & k0;
-                                        c1 ^= k0;
-                                        c2 ^= k1;
-                                    }
+                            WT lt = 0, und = m0 | m1 | m2;
+                            for (int qd = 0; qd < 8 && und; ++qd) {
+                                uint32_t out[4];
+                                philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd, a.key0, a.key1, out);
+    #pragma unroll
+                                for (int t = 0; t < 4; ++t) {
+                                    const WT Rb = rword<WT>(out, t);
+                                    const int sh = 31 - (qd * 4 + t);
+                                    const WT tb = (((th_s[0] >> sh) & 1u) ? m0 : WT(0)) |
+                                                  (((th_s[1] >> sh) & 1u) ? m1 : WT(0)) |
+                                                  (((th_s[2] >> sh) & 1u) ? m2 : WT(0));
+                                    lt |= und & ~Rb & tb;
+                                    und &= ~(Rb ^ tb);
                                 }
-                                // Trotter classes; bits past W are zero in every class
-                                const WT e0 = ~(uL | uR) & valid, e1 = uL ^ uR, e2 = uL & uR;
-                                WT X;
-                                WT ge[9];  // "count >= k" masks (uniform-degree path)
-                                if (FLD) {
-                                    // per-spin thresholds for (sigma, u_t): sigma = +1 bits are ~S_
-                                    flips = 0;
-                                    X = 0;
-#pragma unroll
-                                    for (int cb = 0; cb < 6; ++cb) {
-                                        const WT eu = (cb % 3) == 0 ? e0 : ((cb % 3) == 1 ? e1 : e2);
-                                        const WT m = (cb < 3 ? ~S_ : S_) & eu;
-                                        flips |= m & ge_nib<WT>(fc.ka, cb, c0, c1, c2);
-                                        X |= m & ge_nib<WT>(fc.ks, cb, c0, c1, c2);
-                                    }
-                                } else if (UZ) {
-                                    // thresholds are wave-uniform: every "count >= k" mask
-                                    // once, then six selections by a uniform index
-                                    ge[0] = ~WT(0);
-                                    ge[1] = c0 | c1 | c2;
-                                    ge[2] = c1 | c2;
-                                    ge[3] = c2 | (c1 & c0);
-                                    ge[4] = c2;
-                                    ge[5] = c2 & (c1 | c0);
-                                    ge[6] = c2 & c1;
-                                    ge[7] = c2 & c1 & c0;
-                                    ge[8] = WT(0);
-                                    flips = (e0 & ge[ka0]) | (e1 & ge[ka1]) | (e2 & ge[ka2]);
-                                    X = (e0 & ge[ks0]) | (e1 & ge[ks1]) | (e2 & ge[ks2]);
-                                } else {
-                                    flips = (e0 & ge_pick((int)(zt & 15), c0, c1, c2)) |
-                                            (e1 & ge_pick((int)((zt >> 8) & 15), c0, c1, c2)) |
-                                            (e2 & ge_pick((int)((zt >> 16) & 15), c0, c1, c2));
-                                    X = (e0 & ge_pick((int)((zt >> 4) & 15), c0, c1, c2)) |
-                                        (e1 & ge_pick((int)((zt >> 12) & 15), c0, c1, c2)) |
-                                        (e2 & ge_pick((int)((zt >> 20) & 15), c0, c1, c2));
-                                }
-                                X &= ~flips;
-                                if (CQ && cq) {
-                                    // masks of the step's uniform classes, queued with the word
-                                    // (a single class holds every stochastic bit)
-                                    qX = X;
-                                    if (un == 1) qm0 = X;
-                                    else
-#pragma unroll
-                                    for (int t = 0; t < 3; ++t) {
-                                        if (t < un) {
-                                            const int ut = ut_s[t], us = us_s[t];
-                                            const WT eu = ut == 0 ? e0 : (ut == 1 ? e1 : e2);
-                                            const WT mk = X & eu & ge[us] & ~ge[us + 1];
-                                            if (t == 0) qm0 = mk;
-                                            else if (t == 1) qm1 = mk;
-                                            else qm2 = mk;
+                            }
+                            st[((size_t)rr * 2 + tau) * N + i] ^= lt;
+                        }
+                        qn -= cnt;
+                        __builtin_amdgcn_wave_barrier();
+                    };
+                    // one spin (position q in color order, index i, neighbours nb)
+                    // for every replica of the workgroup; lanes with act false take
+                    // part only in the queue's wave-wide steps
+                    auto spin = [&](const bool act, const int q, const int i, const uint32_t info, const int zi,
+                                    const int *nb) __attribute__((always_inline)) {
+                            const uint32_t zt = UZ ? zt_u : ((PMJ && !FLD) ? tz[zi] : 0u);
+                            // fields: this spin's class thresholds, shared by the replicas below
+                            double hfield = 0.0;
+                            FieldClasses fc;
+                            if (FLD) {
+                                hfield = (double)a.hq[q];
+                                field_classes(hfield, zi, jp, Ts, fc);
+                            }
+                            for (int rr = 0; rr < nrep; ++rr) {
+                              WT qX = 0, qm0 = 0, qm1 = 0, qm2 = 0;  // stochastic word for the queue
+                              if (act) {
+                                WT *mine = st + ((size_t)rr * 2 + tau) * N;
+                                const WT *other = st + ((size_t)rr * 2 + 1 - tau) * N;
+                                const uint32_t rep = a.replica_base + (uint32_t)(r0 + rr);
+                                const WT S_ = mine[i], O = other[i];
+                                // Trotter neighbours of slice 2m+tau: bits m of O and of O
+                                // rotated by one slice (left for tau 0, right for tau 1);
+                                // everything below is symmetric in the two
+                                const WT Lw = O, Rw = ((O << tsa) | (O >> tsb)) & valid;
+                                const WT uL = S_ ^ Lw, uR = S_ ^ Rw;  // 1 = antiparallel Trotter bond
+                                WT flips = 0;
+                                if (PMJ) {
+                                    WT c0 = 0, c1 = 0, c2 = 0;  // bit-sliced unsatisfied-bond counter
+    #pragma unroll
+                                    for (int z = 0; z < 8; ++z) {
+                                        if (ZU > 0 ? z < ZU : (z < a.zmax && z < zi)) {
+                                            const WT x = S_ ^ mine[nb[z]] ^ (((info >> z) & 1u) ? ~WT(0) : WT(0));
+                                            const WT k0 = c0 & x;
+                                            c0 ^= x;
+                                            const WT k1 = c1 & k0;
+                                            c1 ^= k0;
+                                            c2 ^= k1;
                                         }
                                     }
-                                } else if (X) {
-                                    // Bit-sliced comparison u < thr for every stochastic bit at
-                                    // once, most significant uniform bit first: random word R_b
-                                    // supplies bit 31-b of all uniforms; a bit is decided at the
-                                    // first b where its uniform bit differs from its threshold bit.
-                                    WT lt = 0, und = X;
-                                    // The stochastic classes of this word (one per field step on
-                                    // a regular lattice, rarely more): masks and thresholds in
-                                    // registers; more classes take the slow paths below.
-                                    WT cm0 = 0, cm1 = 0, cm2 = 0;
-                                    uint32_t ct0 = 0, ct1 = 0, ct2 = 0;
-                                    int ns = 0;
-                                    if (UZ && !FLD) {
-                                        // the step's uniform classes: masks by indexed selection
-                                        ns = un;
-#pragma unroll
+                                    // Trotter classes; bits past W are zero in every class
+                                    const WT e0 = ~(uL | uR) & valid, e1 = uL ^ uR, e2 = uL & uR;
+                                    WT X;
+                                    WT ge[9];  // "count >= k" masks (uniform-degree path)
+                                    if (FLD) {
+                                        // per-spin thresholds for (sigma, u_t): sigma = +1 bits are ~S_
+                                        flips = 0;
+                                        X = 0;
+    #pragma unroll
+                                        for (int cb = 0; cb < 6; ++cb) {
+                                            const WT eu = (cb % 3) == 0 ? e0 : ((cb % 3) == 1 ? e1 : e2);
+                                            const WT m = (cb < 3 ? ~S_ : S_) & eu;
+                                            flips |= m & ge_nib<WT>(fc.ka, cb, c0, c1, c2);
+                                            X |= m & ge_nib<WT>(fc.ks, cb, c0, c1, c2);
+                                        }
+                                    } else if (UZ) {
+                                        // thresholds are wave-uniform: every "count >= k" mask
+                                        // once, then six selections by a uniform index
+                                        ge[0] = ~WT(0);
+                                        ge[1] = c0 | c1 | c2;
+                                        ge[2] = c1 | c2;
+                                        ge[3] = c2 | (c1 & c0);
+                                        ge[4] = c2;
+                                        ge[5] = c2 & (c1 | c0);
+                                        ge[6] = c2 & c1;
+                                        ge[7] = c2 & c1 & c0;
+                                        ge[8] = WT(0);
+                                        flips = (e0 & ge[ka0]) | (e1 & ge[ka1]) | (e2 & ge[ka2]);
+                                        X = DET ? WT(0) : ((e0 & ge[ks0]) | (e1 & ge[ks1]) | (e2 & ge[ks2]));
+                                    } else {
+                                        flips = (e0 & ge_pick((int)(zt & 15), c0, c1, c2)) |
+                                                (e1 & ge_pick((int)((zt >> 8) & 15), c0, c1, c2)) |
+                                                (e2 & ge_pick((int)((zt >> 16) & 15), c0, c1, c2));
+                                        X = (e0 & ge_pick((int)((zt >> 4) & 15), c0, c1, c2)) |
+                                            (e1 & ge_pick((int)((zt >> 12) & 15), c0, c1, c2)) |
+                                            (e2 & ge_pick((int)((zt >> 20) & 15), c0, c1, c2));
+                                    }
+                                    X &= ~flips;
+                                    if (CQ && cq) {
+                                        // masks of the step's uniform classes, queued with the word
+                                        // (a single class holds every stochastic bit)
+                                        qX = X;
+                                        if (un == 1) qm0 = X;
+                                        else
+    #pragma unroll
                                         for (int t = 0; t < 3; ++t) {
-                                            if (t < ns) {
+                                            if (t < un) {
                                                 const int ut = ut_s[t], us = us_s[t];
                                                 const WT eu = ut == 0 ? e0 : (ut == 1 ? e1 : e2);
-                                                const WT mk = eu & ge[us] & ~ge[us + 1];
-                                                if (t == 0) { cm0 = mk; ct0 = th_s[0]; }
-                                                else if (t == 1) { cm1 = mk; ct1 = th_s[1]; }
-                                                else { cm2 = mk; ct2 = th_s[2]; }
+                                                const WT mk = X & eu & ge[us] & ~ge[us + 1];
+                                                if (t == 0) qm0 = mk;
+                                                else if (t == 1) qm1 = mk;
+                                                else qm2 = mk;
                                             }
                                         }
-                                    } else if (FLD) {
-                                        ns = fc.ns;
-#pragma unroll
-                                        for (int t = 0; t < 3; ++t) {
-                                            if (t < ns) {
-                                                const uint32_t sb = (fc.slot >> (8 * t)) & 255u;
-                                                const int cb = (int)(sb & 7u), us = (int)(sb >> 4);
-                                                const WT eu = (cb % 3) == 0 ? e0 : ((cb % 3) == 1 ? e1 : e2);
-                                                const uint32_t kk = (uint32_t)us | ((uint32_t)(us + 1) << 4);
-                                                const WT mk = (cb < 3 ? ~S_ : S_) & eu & ge_nib<WT>(kk, 0, c0, c1, c2) &
-                                                              ~ge_nib<WT>(kk, 1, c0, c1, c2);
-                                                if (t == 0) { cm0 = mk; ct0 = fc.st[0]; }
-                                                else if (t == 1) { cm1 = mk; ct1 = fc.st[1]; }
-                                                else { cm2 = mk; ct2 = fc.st[2]; }
-                                            }
-                                        }
-                                    } else {
-#pragma unroll
-                                        for (int ut = 0; ut < 3; ++ut) {
-                                            const WT eu = ut == 0 ? e0 : (ut == 1 ? e1 : e2);
-                                            const int amin = (int)((zt >> (8 * ut)) & 15);
-                                            for (int us = (int)((zt >> (8 * ut + 4)) & 15); us < amin; ++us) {
-                                                const WT mk = eu & ge_pick(us, c0, c1, c2) & ~ge_pick(us + 1, c0, c1, c2);
-                                                const uint32_t tk = tthr[(zi * 8 + us) * 3 + ut];
-                                                if (ns == 0) { cm0 = mk; ct0 = tk; }
-                                                else if (ns == 1) { cm1 = mk; ct1 = tk; }
-                                                else if (ns == 2) { cm2 = mk; ct2 = tk; }
-                                                ++ns;
-                                            }
-                                        }
-                                    }
-                                    constexpr int kPer = sizeof(WT) == 4 ? 4 : 2;  // words per Philox call
-                                    if (FLD && ns > 3) {
-                                        // many stochastic triples (high temperature): each
-                                        // stochastic bit assembles its own uniform and threshold
-                                        for (int mm = 0; mm < W; ++mm) {
-                                            if (!((X >> mm) & WT(1))) continue;
-                                            const int sg = (int)((S_ >> mm) & WT(1));
-                                            const int ut = (int)((uL >> mm) & WT(1)) + (int)((uR >> mm) & WT(1));
-                                            const int us = (int)((c0 >> mm) & WT(1)) + 2 * (int)((c1 >> mm) & WT(1)) +
-                                                           4 * (int)((c2 >> mm) & WT(1));
-                                            unsigned int thr = 0;
-                                            (void)field_class(sg ? -hfield : hfield, zi, us,
-                                                              (double)jp * (double)(2 - 2 * ut), Ts, &thr);
-                                            uint32_t u = 0;
-                                            for (int qd = 0; qd < 32 / kPer; ++qd) {
-                                                uint32_t out[4];
-                                                philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd,
-                                                              a.key0, a.key1, out);
-#pragma unroll
-                                                for (int t = 0; t < kPer; ++t)
-                                                    u |= (uint32_t)((rword<WT>(out, t) >> mm) & WT(1)) << (31 - (qd * kPer + t));
-                                            }
-                                            if (u < thr) lt |= WT(1) << mm;
-                                        }
-                                        und = 0;
-                                    }
-                                    for (int qd = 0; qd < 32 / kPer && und; ++qd) {
-                                        uint32_t out[4];
-                                        philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd,
-                                                      a.key0, a.key1, out);
-#pragma unroll
-                                        for (int t = 0; t < kPer; ++t) {
-                                            const WT Rb = rword<WT>(out, t);
-                                            const int sh = 31 - (qd * kPer + t);
-                                            WT tb;  // bits whose class threshold has bit sh set
-                                            if (ns <= 3) {
-                                                tb = (((ct0 >> sh) & 1u) ? cm0 : WT(0)) | (((ct1 >> sh) & 1u) ? cm1 : WT(0)) |
-                                                     (((ct2 >> sh) & 1u) ? cm2 : WT(0));
-                                            } else {
-                                                tb = 0;
-#pragma unroll
-                                                for (int ut = 0; ut < 3; ++ut) {
+                                    } else if (X) {
+                                        // Bit-sliced comparison u < thr for every stochastic bit at
+                                        // once, most significant uniform bit first: random word R_b
+                                        // supplies bit 31-b of all uniforms; a bit is decided at the
+                                        // first b where its uniform bit differs from its threshold bit.
+                                        WT lt = 0, und = X;
+                                        // The stochastic classes of this word (one per field step on
+                                        // a regular lattice, rarely more): masks and thresholds in
+                                        // registers; more classes take the slow paths below.
+                                        WT cm0 = 0, cm1 = 0, cm2 = 0;
+                                        uint32_t ct0 = 0, ct1 = 0, ct2 = 0;
+                                        int ns = 0;
+                                        if (UZ && !FLD) {
+                                            // the step's uniform classes: masks by indexed selection
+                                            ns = un;
+    #pragma unroll
+                                            for (int t = 0; t < 3; ++t) {
+                                                if (t < ns) {
+                                                    const int ut = ut_s[t], us = us_s[t];
                                                     const WT eu = ut == 0 ? e0 : (ut == 1 ? e1 : e2);
-                                                    const int amin = (int)((zt >> (8 * ut)) & 15);
-                                                    for (int us = (int)((zt >> (8 * ut + 4)) & 15); us < amin; ++us)
-                                                        if ((tthr[(zi * 8 + us) * 3 + ut] >> sh) & 1u)
-                                                            tb |= eu & ge_pick(us, c0, c1, c2) & ~ge_pick(us + 1, c0, c1, c2);
+                                                    const WT mk = eu & ge[us] & ~ge[us + 1];
+                                                    if (t == 0) { cm0 = mk; ct0 = th_s[0]; }
+                                                    else if (t == 1) { cm1 = mk; ct1 = th_s[1]; }
+                                                    else { cm2 = mk; ct2 = th_s[2]; }
                                                 }
                                             }
-                                            lt |= und & ~Rb & tb;
-                                            und &= ~(Rb ^ tb);
-                                        }
-                                    }
-                                    flips |= lt & X;
-                                }
-                            } else {
-                                const int2 *row = a.rows + (size_t)i * a.Z;
-                                bool have_r = false;
-                                WT R[32];
-                                for (int mm = 0; mm < W; ++mm) {
-                                    const double si = ((S_ >> mm) & 1) ? -1.0 : 1.0;
-                                    double h = 0.0;
-                                    for (int e = 0; e < a.Z; ++e) {
-                                        const int2 rw = row[e];
-                                        const double J = (double)__int_as_float(rw.y);
-                                        if (rw.x == i) h += J;
-                                        else h += J * (((mine[rw.x] >> mm) & 1) ? -1.0 : 1.0);
-                                    }
-                                    const double ts = (((Lw >> mm) & 1) ? -1.0 : 1.0) + (((Rw >> mm) & 1) ? -1.0 : 1.0);
-                                    const double ed = -2.0 * si * (h + (double)jp * ts);
-                                    unsigned int thr = 0;
-                                    const int cls = piqmc_fixed_class(ed, Ts, &thr);
-                                    bool fl = cls == 1;
-                                    if (cls == 2) {
-                                        if (!have_r) {  // the word's 32 bit-sliced random words
-                                            have_r = true;
-                                            constexpr int kPer = sizeof(WT) == 4 ? 4 : 2;
-#pragma unroll
-                                            for (int qd = 0; qd < 32 / kPer; ++qd) {
-                                                uint32_t out[4];
-                                                philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd,
-                                                              a.key0, a.key1, out);
-#pragma unroll
-                                                for (int t = 0; t < kPer; ++t)
-                                                    R[qd * kPer + t] = rword<WT>(out, t);
+                                        } else if (FLD) {
+                                            ns = fc.ns;
+    #pragma unroll
+                                            for (int t = 0; t < 3; ++t) {
+                                                if (t < ns) {
+                                                    const uint32_t sb = (fc.slot >> (8 * t)) & 255u;
+                                                    const int cb = (int)(sb & 7u), us = (int)(sb >> 4);
+                                                    const WT eu = (cb % 3) == 0 ? e0 : ((cb % 3) == 1 ? e1 : e2);
+                                                    const uint32_t kk = (uint32_t)us | ((uint32_t)(us + 1) << 4);
+                                                    const WT mk = (cb < 3 ? ~S_ : S_) & eu & ge_nib<WT>(kk, 0, c0, c1, c2) &
+                                                                  ~ge_nib<WT>(kk, 1, c0, c1, c2);
+                                                    if (t == 0) { cm0 = mk; ct0 = fc.st[0]; }
+                                                    else if (t == 1) { cm1 = mk; ct1 = fc.st[1]; }
+                                                    else { cm2 = mk; ct2 = fc.st[2]; }
+                                                }
+                                            }
+                                        } else {
+    #pragma unroll
+                                            for (int ut = 0; ut < 3; ++ut) {
+                                                const WT eu = ut == 0 ? e0 : (ut == 1 ? e1 : e2);
+                                                const int amin = (int)((zt >> (8 * ut)) & 15);
+                                                for (int us = (int)((zt >> (8 * ut + 4)) & 15); us < amin; ++us) {
+                                                    const WT mk = eu & ge_pick(us, c0, c1, c2) & ~ge_pick(us + 1, c0, c1, c2);
+                                                    const uint32_t tk = tthr[(zi * 8 + us) * 3 + ut];
+                                                    if (ns == 0) { cm0 = mk; ct0 = tk; }
+                                                    else if (ns == 1) { cm1 = mk; ct1 = tk; }
+                                                    else if (ns == 2) { cm2 = mk; ct2 = tk; }
+                                                    ++ns;
+                                                }
                                             }
                                         }
-                                        uint32_t u = 0;
-#pragma unroll
-                                        for (int bb = 0; bb < 32; ++bb) u |= (uint32_t)((R[bb] >> mm) & 1) << (31 - bb);
-                                        fl = u < thr;
+                                        constexpr int kPer = sizeof(WT) == 4 ? 4 : 2;  // words per Philox call
+                                        if (FLD && ns > 3) {
+                                            // many stochastic triples (high temperature): each
+                                            // stochastic bit assembles its own uniform and threshold
+                                            for (int mm = 0; mm < W; ++mm) {
+                                                if (!((X >> mm) & WT(1))) continue;
+                                                const int sg = (int)((S_ >> mm) & WT(1));
+                                                const int ut = (int)((uL >> mm) & WT(1)) + (int)((uR >> mm) & WT(1));
+                                                const int us = (int)((c0 >> mm) & WT(1)) + 2 * (int)((c1 >> mm) & WT(1)) +
+                                                               4 * (int)((c2 >> mm) & WT(1));
+                                                unsigned int thr = 0;
+                                                (void)field_class(sg ? -hfield : hfield, zi, us,
+                                                                  (double)jp * (double)(2 - 2 * ut), Ts, &thr);
+                                                uint32_t u = 0;
+                                                for (int qd = 0; qd < 32 / kPer; ++qd) {
+                                                    uint32_t out[4];
+                                                    philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd,
+                                                                  a.key0, a.key1, out);
+    #pragma unroll
+                                                    for (int t = 0; t < kPer; ++t)
+                                                        u |= (uint32_t)((rword<WT>(out, t) >> mm) & WT(1)) << (31 - (qd * kPer + t));
+                                                }
+                                                if (u < thr) lt |= WT(1) << mm;
+                                            }
+                                            und = 0;
+                                        }
+                                        for (int qd = 0; qd < 32 / kPer && und; ++qd) {
+                                            uint32_t out[4];
+                                            philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd,
+                                                          a.key0, a.key1, out);
+    #pragma unroll
+                                            for (int t = 0; t < kPer; ++t) {
+                                                const WT Rb = rword<WT>(out, t);
+                                                const int sh = 31 - (qd * kPer + t);
+                                                WT tb;  // bits whose class threshold has bit sh set
+                                                if (ns <= 3) {
+                                                    tb = (((ct0 >> sh) & 1u) ? cm0 : WT(0)) | (((ct1 >> sh) & 1u) ? cm1 : WT(0)) |
+                                                         (((ct2 >> sh) & 1u) ? cm2 : WT(0));
+                                                } else {
+                                                    tb = 0;
+    #pragma unroll
+                                                    for (int ut = 0; ut < 3; ++ut) {
+                                                        const WT eu = ut == 0 ? e0 : (ut == 1 ? e1 : e2);
+                                                        const int amin = (int)((zt >> (8 * ut)) & 15);
+                                                        for (int us = (int)((zt >> (8 * ut + 4)) & 15); us < amin; ++us)
+                                                            if ((tthr[(zi * 8 + us) * 3 + ut] >> sh) & 1u)
+                                                                tb |= eu & ge_pick(us, c0, c1, c2) & ~ge_pick(us + 1, c0, c1, c2);
+                                                    }
+                                                }
+                                                lt |= und & ~Rb & tb;
+                                                und &= ~(Rb ^ tb);
+                                            }
+                                        }
+                                        flips |= lt & X;
                                     }
-                                    if (fl) flips |= WT(1) << mm;
+                                } else {
+                                    const int2 *row = a.rows + (size_t)i * a.Z;
+                                    bool have_r = false;
+                                    WT R[32];
+                                    for (int mm = 0; mm < W; ++mm) {
+                                        const double si = ((S_ >> mm) & 1) ? -1.0 : 1.0;
+                                        double h = 0.0;
+                                        for (int e = 0; e < a.Z; ++e) {
+                                            const int2 rw = row[e];
+                                            const double J = (double)__int_as_float(rw.y);
+                                            if (rw.x == i) h += J;
+                                            else h += J * (((mine[rw.x] >> mm) & 1) ? -1.0 : 1.0);
+                                        }
+                                        const double ts = (((Lw >> mm) & 1) ? -1.0 : 1.0) + (((Rw >> mm) & 1) ? -1.0 : 1.0);
+                                        const double ed = -2.0 * si * (h + (double)jp * ts);
+                                        unsigned int thr = 0;
+                                        const int cls = piqmc_fixed_class(ed, Ts, &thr);
+                                        bool fl = cls == 1;
+                                        if (cls == 2) {
+                                            if (!have_r) {  // the word's 32 bit-sliced random words
+                                                have_r = true;
+                                                constexpr int kPer = sizeof(WT) == 4 ? 4 : 2;
+    #pragma unroll
+                                                for (int qd = 0; qd < 32 / kPer; ++qd) {
+                                                    uint32_t out[4];
+                                                    philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd,
+                                                                  a.key0, a.key1, out);
+    #pragma unroll
+                                                    for (int t = 0; t < kPer; ++t)
+                                                        R[qd * kPer + t] = rword<WT>(out, t);
+                                                }
+                                            }
+                                            uint32_t u = 0;
+    #pragma unroll
+                                            for (int bb = 0; bb < 32; ++bb) u |= (uint32_t)((R[bb] >> mm) & 1) << (31 - bb);
+                                            fl = u < thr;
+                                        }
+                                        if (fl) flips |= WT(1) << mm;
+                                    }
                                 }
-                            }
-                            mine[i] = S_ ^ flips;
-                          }
-                          if constexpr (CQ) {
-                              if (cq) {
-                                  // append this lane's stochastic word (if any) to the
-                                  // wavefront's queue; a full batch of 64 is drained at once
-                                  const unsigned long long bal = __ballot(qX != 0);
-                                  if (qX != 0) {
-                                      const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi(
-                                                               (uint32_t)(bal >> 32),
-                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                                      if (un == 1)
-                                          *reinterpret_cast<uint2 *>(wq + pos) =
-                                              make_uint2((uint32_t)i | ((uint32_t)rr << 16), (uint32_t)qm0);
-                                      else
-                                          wq[pos] = make_uint4((uint32_t)i | ((uint32_t)rr << 16), (uint32_t)qm0,
-                                                               (uint32_t)qm1, (uint32_t)qm2);
-                                  }
-                                  qn += (int)__popcll(bal);
-                                  if (qn >= 64) {
-                                      if (un == 1) drain(64, std::integral_constant<int, 1>());
-                                      else drain(64, std::integral_constant<int, 3>());
+                                mine[i] = S_ ^ flips;
+                              }
+                              if constexpr (CQ && !DET) {
+                                  if (cq) {
+                                      // append this lane's stochastic word (if any) to the
+                                      // wavefront's queue; a full batch of 64 is drained at once
+                                      const unsigned long long bal = __ballot(qX != 0);
+                                      if (qX != 0) {
+                                          const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi(
+                                                                   (uint32_t)(bal >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                                          if (un == 1)
+                                              *reinterpret_cast<uint2 *>(wq + pos) =
+                                                  make_uint2((uint32_t)i | ((uint32_t)rr << 16), (uint32_t)qm0);
+                                          else
+                                              wq[pos] = make_uint4((uint32_t)i | ((uint32_t)rr << 16), (uint32_t)qm0,
+                                                                   (uint32_t)qm1, (uint32_t)qm2);
+                                      }
+                                      qn += (int)__popcll(bal);
+                                      if (qn >= 64) {
+                                          if (un == 1) drain(64, std::integral_constant<int, 1>());
+                                          else drain(64, std::integral_constant<int, 3>());
+                                      }
                                   }
                               }
-                          }
-                        }
-                };
-                if constexpr (CACHED) {
-#pragma unroll
-                    for (int c = 0; c < 2; ++c) {
-#pragma unroll
-                        for (int k = 0; k < 2; ++k) {
-                            // (c, k) are compile-time after unrolling: the cached registers directly
-                            const bool c0k = c == 0, k0k = k == 0;
-                            const int q = c0k ? (k0k ? cq_[0][0] : cq_[0][1]) : (k0k ? cq_[1][0] : cq_[1][1]);
-                            if (!CQ && q < 0) continue;
-                            const int i = c0k ? (k0k ? cI[0][0] : cI[0][1]) : (k0k ? cI[1][0] : cI[1][1]);
-                            const uint32_t info = c0k ? (k0k ? cF[0][0] : cF[0][1]) : (k0k ? cF[1][0] : cF[1][1]);
-                            int nb[ZC];
-#pragma unroll
-                            for (int z = 0; z < ZC; ++z)
-                                nb[z] = c0k ? (k0k ? cN[0][0][z] : cN[0][1][z]) : (k0k ? cN[1][0][z] : cN[1][1][z]);
-                            spin(q >= 0, q, i, info, ZU, nb);
-                        }
-                        if (CQ && qn > 0) {
-                            if (un == 1) drain(qn, std::integral_constant<int, 1>());
-                            else drain(qn, std::integral_constant<int, 3>());
-                        }
-                        __syncthreads();
-                    }
-                } else {
-                    for (int c = 0; c < a.ncolors; ++c) {
-                        const int q0 = a.color_off[c], q1 = a.color_off[c + 1];
-                        for (int qb = q0; qb < q1; qb += bd) {
-                            // with the queue every lane runs every pass (inactive past q1)
-                            const bool act = qb + tid < q1;
-                            if (!CQ && !act) break;
-                            const int q = act ? qb + tid : q0;
-                            const int i = a.order[q];
-                            uint32_t info = 0;
-                            int zi = 0;
-                            int nb[8];
-                            if (PMJ) {
-                                info = a.cinfo[q];
-                                zi = ZU > 0 ? ZU : (UZ ? a.zuni : (int)((info >> 8) & 15));
-                                if (ZU > 0) {
-#pragma unroll
-                                    for (int z = 0; z < ZU; ++z) nb[z] = a.cnb[(size_t)z * N + q];
-                                } else {
-#pragma unroll
-                                    for (int z = 0; z < 8; ++z)
-                                        if (z < a.zmax) nb[z] = z < zi ? a.cnb[(size_t)z * N + q] : i;
-                                }
                             }
-                            spin(act, q, i, info, zi, nb);
+                    };
+                    if constexpr (CACHED) {
+    #pragma unroll
+                        for (int c = 0; c < 2; ++c) {
+    #pragma unroll
+                            for (int k = 0; k < 2; ++k) {
+                                // (c, k) are compile-time after unrolling: the cached registers directly
+                                const bool c0k = c == 0, k0k = k == 0;
+                                const int q = c0k ? (k0k ? cq_[0][0] : cq_[0][1]) : (k0k ? cq_[1][0] : cq_[1][1]);
+                                if (!CQ && q < 0) continue;
+                                const int i = c0k ? (k0k ? cI[0][0] : cI[0][1]) : (k0k ? cI[1][0] : cI[1][1]);
+                                const uint32_t info = c0k ? (k0k ? cF[0][0] : cF[0][1]) : (k0k ? cF[1][0] : cF[1][1]);
+                                int nb[ZC];
+    #pragma unroll
+                                for (int z = 0; z < ZC; ++z)
+                                    nb[z] = c0k ? (k0k ? cN[0][0][z] : cN[0][1][z]) : (k0k ? cN[1][0][z] : cN[1][1][z]);
+                                spin(q >= 0, q, i, info, ZU, nb);
+                            }
+                            if (CQ && !DET && qn > 0) {
+                                if (un == 1) drain(qn, std::integral_constant<int, 1>());
+                                else drain(qn, std::integral_constant<int, 3>());
+                            }
+                            __syncthreads();
                         }
-                        if (CQ && qn > 0) {
-                            if (un == 1) drain(qn, std::integral_constant<int, 1>());
-                            else drain(qn, std::integral_constant<int, 3>());
+                    } else {
+                        for (int c = 0; c < a.ncolors; ++c) {
+                            const int q0 = a.color_off[c], q1 = a.color_off[c + 1];
+                            for (int qb = q0; qb < q1; qb += bd) {
+                                // with the queue every lane runs every pass (inactive past q1)
+                                const bool act = qb + tid < q1;
+                                if (!CQ && !act) break;
+                                const int q = act ? qb + tid : q0;
+                                const int i = a.order[q];
+                                uint32_t info = 0;
+                                int zi = 0;
+                                int nb[8];
+                                if (PMJ) {
+                                    info = a.cinfo[q];
+                                    zi = ZU > 0 ? ZU : (UZ ? a.zuni : (int)((info >> 8) & 15));
+                                    if (ZU > 0) {
+    #pragma unroll
+                                        for (int z = 0; z < ZU; ++z) nb[z] = a.cnb[(size_t)z * N + q];
+                                    } else {
+    #pragma unroll
+                                        for (int z = 0; z < 8; ++z)
+                                            if (z < a.zmax) nb[z] = z < zi ? a.cnb[(size_t)z * N + q] : i;
+                                    }
+                                }
+                                spin(act, q, i, info, zi, nb);
+                            }
+                            if (CQ && !DET && qn > 0) {
+                                if (un == 1) drain(qn, std::integral_constant<int, 1>());
+                                else drain(qn, std::integral_constant<int, 3>());
+                            }
+                            __syncthreads();
                         }
-                        __syncthreads();
                     }
                 }
             }
+        };
+        if constexpr (CQ) {
+            if (un == 0) sweeps(std::true_type());
+            else sweeps(std::false_type());
+        } else {
+            sweeps(std::false_type());
         }
         if (a.etrace) {
             // Mean over slices of the classical energy; sums of +-1 products are
