@@ -273,10 +273,15 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
         const float jp = a.jperp[s];
         const double Ts = (double)a.temps[s];
         const bool cq = CQ && un <= 3;  // this step's stochastic words go through the queue
-        // Steps without stochastic classes (CQ kernels) run a copy of the
-        // sweep with every stochastic branch compiled out.
-        auto sweeps = [&](auto DETc) __attribute__((always_inline)) {
-            constexpr bool DET = decltype(DETc)::value;
+        // CQ kernels run one of three copies of the sweep, chosen per step:
+        // MODE 0 without stochastic classes (every stochastic branch compiled
+        // out), MODE 1 with exactly one (queue records are the stochastic
+        // mask), MODE 2 otherwise.
+        auto sweeps = [&](auto MODEc) __attribute__((always_inline)) {
+            constexpr int MODE = decltype(MODEc)::value;
+            constexpr bool DET = MODE == 0;
+            const bool one = MODE == 1 || (MODE == 2 && un == 1);  // a single stochastic class
+            const bool cqm = MODE == 1 || cq;
             for (int m = 0; m < a.M; ++m) {
                 const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
                 for (int tau = 0; tau < 2; ++tau) {
@@ -397,11 +402,11 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                             (e2 & ge_pick((int)((zt >> 20) & 15), c0, c1, c2));
                                     }
                                     X &= ~flips;
-                                    if (CQ && cq) {
+                                    if (CQ && cqm) {
                                         // masks of the step's uniform classes, queued with the word
                                         // (a single class holds every stochastic bit)
                                         qX = X;
-                                        if (un == 1) qm0 = X;
+                                        if (one) qm0 = X;
                                         else
     #pragma unroll
                                         for (int t = 0; t < 3; ++t) {
@@ -569,7 +574,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 mine[i] = S_ ^ flips;
                               }
                               if constexpr (CQ && !DET) {
-                                  if (cq) {
+                                  if (cqm) {
                                       // append this lane's stochastic word (if any) to the
                                       // wavefront's queue; a full batch of 64 is drained at once
                                       const unsigned long long bal = __ballot(qX != 0);
@@ -577,7 +582,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                           const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi(
                                                                    (uint32_t)(bal >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                                          if (un == 1)
+                                          if (one)
                                               *reinterpret_cast<uint2 *>(wq + pos) =
                                                   make_uint2((uint32_t)i | ((uint32_t)rr << 16), (uint32_t)qm0);
                                           else
@@ -586,7 +591,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                       }
                                       qn += (int)__popcll(bal);
                                       if (qn >= 64) {
-                                          if (un == 1) drain(64, std::integral_constant<int, 1>());
+                                          if (one) drain(64, std::integral_constant<int, 1>());
                                           else drain(64, std::integral_constant<int, 3>());
                                       }
                                   }
@@ -611,7 +616,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 spin(q >= 0, q, i, info, ZU, nb);
                             }
                             if (CQ && !DET && qn > 0) {
-                                if (un == 1) drain(qn, std::integral_constant<int, 1>());
+                                if (one) drain(qn, std::integral_constant<int, 1>());
                                 else drain(qn, std::integral_constant<int, 3>());
                             }
                             __syncthreads();
@@ -643,7 +648,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 spin(act, q, i, info, zi, nb);
                             }
                             if (CQ && !DET && qn > 0) {
-                                if (un == 1) drain(qn, std::integral_constant<int, 1>());
+                                if (one) drain(qn, std::integral_constant<int, 1>());
                                 else drain(qn, std::integral_constant<int, 3>());
                             }
                             __syncthreads();
@@ -653,10 +658,11 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
             }
         };
         if constexpr (CQ) {
-            if (un == 0) sweeps(std::true_type());
-            else sweeps(std::false_type());
+            if (un == 0) sweeps(std::integral_constant<int, 0>());
+            else if (un == 1) sweeps(std::integral_constant<int, 1>());
+            else sweeps(std::integral_constant<int, 2>());
         } else {
-            sweeps(std::false_type());
+            sweeps(std::integral_constant<int, 2>());
         }
         if (a.etrace) {
             // Mean over slices of the classical energy; sums of +-1 products are
