@@ -705,6 +705,18 @@ __global__ void __attribute__((flatten)) colored_kernel(ColArgs<WT> a)
     colored_body<WT, PMJ, UZ, FLD, ZU, CACHED, CQ>(a, r0, min(a.rpw, a.R - r0), smem);
 }
 
+// The queue kernels (+-1 couplings, uniform degree) with a register budget of
+// five wavefronts per SIMD: left to itself the compiler takes 117 registers
+// and four; on the 1024-spin workload five run 15.1 -> 14.4 ms despite a
+// 12-byte spill.
+template <typename WT, int ZU, bool CACHED>
+__global__ void __attribute__((flatten)) __attribute__((amdgpu_waves_per_eu(5))) colored_q_kernel(ColArgs<WT> a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int r0 = blockIdx.x * a.rpw;
+    colored_body<WT, true, true, false, ZU, CACHED, true>(a, r0, min(a.rpw, a.R - r0), smem);
+}
+
 // Launch of the +-1 sweep for a uniform-degree lattice of compile-time degree
 // ZU, with per-thread cached neighbour lists when two colors of at most two
 // spins per thread allow it, and with the per-wavefront stochastic queue when
@@ -712,14 +724,24 @@ __global__ void __attribute__((flatten)) colored_kernel(ColArgs<WT> a)
 template <typename WT, int ZU, bool CQ>
 void launch_uniform_q(const ColArgs<WT> &a, bool cached, int grid, int block, size_t lds, hipStream_t st)
 {
-    if (cached) {
-        (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true, false, ZU, true, CQ>,
+    if constexpr (CQ) {
+        if (cached) {
+            (void)hipFuncSetAttribute((const void *)colored_q_kernel<WT, ZU, true>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            colored_q_kernel<WT, ZU, true><<<grid, block, lds, st>>>(a);
+        } else {
+            (void)hipFuncSetAttribute((const void *)colored_q_kernel<WT, ZU, false>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            colored_q_kernel<WT, ZU, false><<<grid, block, lds, st>>>(a);
+        }
+    } else if (cached) {
+        (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true, false, ZU, true>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        colored_kernel<WT, true, true, false, ZU, true, CQ><<<grid, block, lds, st>>>(a);
+        colored_kernel<WT, true, true, false, ZU, true><<<grid, block, lds, st>>>(a);
     } else {
-        (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true, false, ZU, false, CQ>,
+        (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true, false, ZU, false>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        colored_kernel<WT, true, true, false, ZU, false, CQ><<<grid, block, lds, st>>>(a);
+        colored_kernel<WT, true, true, false, ZU, false><<<grid, block, lds, st>>>(a);
     }
 }
 
@@ -1455,11 +1477,16 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
             launch_uniform<WT, 6>(a, pb, grid, block, lds, st);
         } else if (a.zuni >= 0) {
             const size_t lq = queue_lds(N, rpw, sizeof(WT), block);
-            if (sizeof(WT) == 4 && lq <= 160 * 1024 && queue_enabled()) {
-                (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true, false, 0, false, sizeof(WT) == 4>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lq);
-                colored_kernel<WT, true, true, false, 0, false, sizeof(WT) == 4><<<grid, block, lq, st>>>(a);
-            } else {
+            bool queued = false;
+            if constexpr (sizeof(WT) == 4) {
+                if (lq <= 160 * 1024 && queue_enabled()) {
+                    (void)hipFuncSetAttribute((const void *)colored_q_kernel<WT, 0, false>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lq);
+                    colored_q_kernel<WT, 0, false><<<grid, block, lq, st>>>(a);
+                    queued = true;
+                }
+            }
+            if (!queued) {
                 (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
                 colored_kernel<WT, true, true><<<grid, block, lds, st>>>(a);
             }
