@@ -32,9 +32,14 @@
 namespace {
 
 constexpr int kClassStride = 8 * 8 * 3;  // (z, u_s, u_t) thresholds per field step
-// Stochastic-word queue of the +-1 sweep: records (index | replica << 16, three
-// class masks) per wavefront; it never holds more than 63 + 64 between drains.
+// Stochastic-word queues of the +-1 sweep, per wavefront, never holding more
+// than 63 + 64 records between drains: with several classes one queue of
+// 16-byte records (index | replica << 16, three class masks); with a single
+// class a queue of 8-byte records (index | replica << 16, mask) and a second
+// one of 16-byte records for words still undecided after their first Philox
+// block (index | replica << 16, mask, accepted bits, undecided bits).
 constexpr int kQueueCap = 128;
+constexpr int kQueueBytes = 1024 + 2048;  // per wavefront: [128] uint2 + [128] uint4, or [128] uint4
 
 // LDS of the word-per-thread sweep: packed state [rpw][2][N] words, class
 // thresholds [kClassStride], per-degree table [8], reduction scratch [16]
@@ -212,10 +217,13 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
     uint32_t *tthr = reinterpret_cast<uint32_t *>(st + (size_t)2 * N * a.rpw);  // [kClassStride]
     uint32_t *tz = tthr + kClassStride;                               // [8]
     double *red = reinterpret_cast<double *>(tz + 8);                 // [16]
-    // per-wavefront queue of stochastic words (CQ): [waves][kQueueCap]
-    uint4 *wq = reinterpret_cast<uint4 *>(smem + queue_offset(a.N, a.rpw, sizeof(WT))) +
-                (size_t)(tid >> 6) * kQueueCap;
+    // per-wavefront queues of stochastic words (CQ), kQueueBytes each
+    unsigned char *wqb = smem + queue_offset(a.N, a.rpw, sizeof(WT)) + (size_t)(tid >> 6) * kQueueBytes;
+    uint4 *wq = reinterpret_cast<uint4 *>(wqb);           // several classes
+    uint2 *wq1 = reinterpret_cast<uint2 *>(wqb);          // single class: fresh words
+    uint4 *wq2 = reinterpret_cast<uint4 *>(wqb + 1024);   // single class: undecided after block 0
     int qn = 0;  // queued records of this wavefront (wave-uniform)
+    int q2 = 0;  // records in wq2 (wave-uniform)
     const WT valid = (W == (int)(8 * sizeof(WT))) ? ~WT(0) : ((WT(1) << W) - 1);
     WT *gst = a.packed + (size_t)r0 * 2 * N;
     for (int q = tid; q < 2 * N * nrep; q += bd) st[q] = gst[q];
@@ -290,6 +298,75 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                     // bits meet the same bit-sliced uniforms, thresholds and early
                     // exit as in the inline path, and the accepted ones are XORed
                     // into the word (its deterministic flips are already stored).
+                    // Single class, second stage: the top cnt records of wq2 continue
+                    // from Philox block 1 with the usual early exit.
+                    auto drain2 = [&](const int cnt) __attribute__((always_inline)) {
+                        __builtin_amdgcn_wave_barrier();
+                        const int lane = tid & 63;
+                        if (lane < cnt) {
+                            const uint4 rec = wq2[q2 - cnt + lane];
+                            const int i = (int)(rec.x & 0xffffu), rr = (int)(rec.x >> 16);
+                            const WT m0 = (WT)rec.y;
+                            WT lt = (WT)rec.z, und = (WT)rec.w;
+                            const uint32_t rep = a.replica_base + (uint32_t)(r0 + rr);
+                            for (int qd = 1; qd < 8 && und; ++qd) {
+                                uint32_t out[4];
+                                philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd, a.key0, a.key1, out);
+    #pragma unroll
+                                for (int t = 0; t < 4; ++t) {
+                                    const WT Rb = rword<WT>(out, t);
+                                    const WT tb = ((th_s[0] >> (31 - (qd * 4 + t))) & 1u) ? m0 : WT(0);
+                                    lt |= und & ~Rb & tb;
+                                    und &= ~(Rb ^ tb);
+                                }
+                            }
+                            st[((size_t)rr * 2 + tau) * N + i] ^= lt;
+                        }
+                        q2 -= cnt;
+                        __builtin_amdgcn_wave_barrier();
+                    };
+                    // Single class, first stage: the top cnt fresh records meet
+                    // Philox block 0 (four bit planes); decided words are written,
+                    // undecided ones move to wq2, which is resolved 64 at a time,
+                    // so a word needing more blocks no longer holds up 63 others.
+                    auto drain1 = [&](const int cnt) __attribute__((always_inline)) {
+                        __builtin_amdgcn_wave_barrier();
+                        const int lane = tid & 63;
+                        bool more = false;
+                        uint4 rec2 = make_uint4(0u, 0u, 0u, 0u);
+                        if (lane < cnt) {
+                            const uint2 r2 = wq1[qn - cnt + lane];
+                            const int i = (int)(r2.x & 0xffffu), rr = (int)(r2.x >> 16);
+                            const WT m0 = (WT)r2.y;
+                            WT lt = 0, und = m0;
+                            uint32_t out[4];
+                            philox4x32_10(g, (uint32_t)i, a.replica_base + (uint32_t)(r0 + rr), uint32_t(tau) << 24,
+                                          a.key0, a.key1, out);
+    #pragma unroll
+                            for (int t = 0; t < 4; ++t) {
+                                const WT Rb = rword<WT>(out, t);
+                                const WT tb = ((th_s[0] >> (31 - t)) & 1u) ? m0 : WT(0);
+                                lt |= und & ~Rb & tb;
+                                und &= ~(Rb ^ tb);
+                            }
+                            if (und == 0) {
+                                st[((size_t)rr * 2 + tau) * N + i] ^= lt;
+                            } else {
+                                more = true;
+                                rec2 = make_uint4(r2.x, r2.y, (uint32_t)lt, (uint32_t)und);
+                            }
+                        }
+                        qn -= cnt;
+                        const unsigned long long bal = __ballot(more);
+                        if (more) {
+                            const int pos = q2 + (int)__builtin_amdgcn_mbcnt_hi(
+                                                     (uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                            wq2[pos] = rec2;
+                        }
+                        q2 += (int)__popcll(bal);
+                        __builtin_amdgcn_wave_barrier();
+                        if (q2 >= 64) drain2(64);
+                    };
                     // NS: classes of the step (1, or up to 3), a compile-time constant.
                     auto drain = [&](const int cnt, auto NS) __attribute__((always_inline)) {
                         __builtin_amdgcn_wave_barrier();
@@ -582,16 +659,15 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                           const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi(
                                                                    (uint32_t)(bal >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                                          if (one)
-                                              *reinterpret_cast<uint2 *>(wq + pos) =
-                                                  make_uint2((uint32_t)i | ((uint32_t)rr << 16), (uint32_t)qm0);
+                                          if (MODE == 1)
+                                              wq1[pos] = make_uint2((uint32_t)i | ((uint32_t)rr << 16), (uint32_t)qm0);
                                           else
                                               wq[pos] = make_uint4((uint32_t)i | ((uint32_t)rr << 16), (uint32_t)qm0,
                                                                    (uint32_t)qm1, (uint32_t)qm2);
                                       }
                                       qn += (int)__popcll(bal);
                                       if (qn >= 64) {
-                                          if (one) drain(64, std::integral_constant<int, 1>());
+                                          if constexpr (MODE == 1) drain1(64);
                                           else drain(64, std::integral_constant<int, 3>());
                                       }
                                   }
@@ -615,9 +691,11 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                     nb[z] = c0k ? (k0k ? cN[0][0][z] : cN[0][1][z]) : (k0k ? cN[1][0][z] : cN[1][1][z]);
                                 spin(q >= 0, q, i, info, ZU, nb);
                             }
-                            if (CQ && !DET && qn > 0) {
-                                if (one) drain(qn, std::integral_constant<int, 1>());
-                                else drain(qn, std::integral_constant<int, 3>());
+                            if constexpr (CQ && MODE == 1) {
+                                if (qn > 0) drain1(qn);
+                                while (q2 > 0) drain2(min(q2, 64));
+                            } else if (CQ && !DET && qn > 0) {
+                                drain(qn, std::integral_constant<int, 3>());
                             }
                             __syncthreads();
                         }
@@ -647,9 +725,11 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 }
                                 spin(act, q, i, info, zi, nb);
                             }
-                            if (CQ && !DET && qn > 0) {
-                                if (one) drain(qn, std::integral_constant<int, 1>());
-                                else drain(qn, std::integral_constant<int, 3>());
+                            if constexpr (CQ && MODE == 1) {
+                                if (qn > 0) drain1(qn);
+                                while (q2 > 0) drain2(min(q2, 64));
+                            } else if (CQ && !DET && qn > 0) {
+                                drain(qn, std::integral_constant<int, 3>());
                             }
                             __syncthreads();
                         }
@@ -764,7 +844,7 @@ inline bool dense_wg_enabled()
 // LDS of the word-per-thread sweep with the stochastic queues of block/64 wavefronts.
 inline size_t queue_lds(int N, int rpw, int word_bytes, int block)
 {
-    return queue_offset(N, rpw, word_bytes) + (size_t)(block / 64) * kQueueCap * sizeof(uint4);
+    return queue_offset(N, rpw, word_bytes) + (size_t)(block / 64) * kQueueBytes;
 }
 
 template <typename WT, int ZU>
