@@ -287,7 +287,7 @@ def main():
                    "replicas": R, "nspins": N, "slices": P, "field_steps": S, "sweeps_per_step": 1},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(R, N, P, S),
-                     "kernel": "colored_kernel<uint32,+-1 path>", "tile": {"block": tile[0], "rpw": tile[1]},
+                     "kernel": "colored_q_kernel<uint32, degree 4, cached neighbours>", "tile": {"block": tile[0], "rpw": tile[1]},
                      "kernel_ms": kavg * 1e3, "kernel_attempts_per_s": k_rate,
                      "bytes_per_attempt": BYTES_PER_ATTEMPT,
                      "lds_frac": achieved / LDS_PEAK_GBS,
