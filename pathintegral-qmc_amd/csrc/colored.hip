@@ -62,6 +62,7 @@ struct ColArgs {
     const int32_t *cnb;        // [zmax][N] (+-1 path)
     const uint32_t *cinfo;     // [N]
     const float *hq;           // [N] field of spin order[q] (+-1 couplings with fields)
+    const double *prevj;       // [N] coupling of spin order[q] to order[q-1] (dense path)
     const uint32_t *thr;       // [S][kClassStride]
     const uint32_t *ztab;      // [S][8] per degree z: u_s thresholds (see run_colored)
     WT *packed;                // [R][2][N]
@@ -1107,6 +1108,7 @@ __device__ void dense_wg_replica(const ColArgs<WT> &a, int r, unsigned char *bas
     const uint32_t rep = a.replica_base + (uint32_t)r;
     WT *gst = a.packed + (size_t)r * 2 * N;
     for (int q = tid; q < 2 * N; q += 256) st[q] = gst[q];
+    __syncthreads();  // the staging below reads spins that other wavefronts loaded
     constexpr int kPre = 2;  // row entries tid + 256 k, read one spin ahead
     int2 pre[kPre];
     auto fetch_row = [&](int i) {
@@ -1255,6 +1257,239 @@ __global__ void __launch_bounds__(256) dense_wg_kernel(ColArgs<WT> a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     dense_wg_replica<WT>(a, blockIdx.x, smem);
+}
+
+// Workgroup barrier that orders LDS accesses only: global loads issued
+// before it (prefetches) stay outstanding instead of being waited for.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The same sweep pipelined over the workgroup's four wavefronts: wavefront 0
+// decides spin order[q] while wavefronts 1-3 stage and sum the row of
+// order[q+1], one barrier per spin.  The entries of that row for order[q]
+// (whose flips are being decided) are left out of the sums, and wavefront 0
+// adds them back, with order[q]'s new spins, before deciding order[q+1]
+// (prevj holds their summed coupling).  Exactly summable rows make every
+// partial and the completed field the exact field, so the doubles, the
+// Philox words and the flips are those of the lane-per-slice body.  Each
+// summing wavefront stages only its own entries (wavefront-private LDS, no
+// barrier between staging and summing); partial sums are double-buffered.
+// A staged entry is one 16-byte record (coupling, spin word) that a lane reads
+// with one LDS instruction; with 64-bit words a second copy carries the high
+// half for lanes 32-63.  Each lane group's records are contiguous.
+__host__ __device__ inline int dense_pipe_slots(int Z) { return (Z + 2) / 3 + 64; }
+__host__ __device__ inline size_t dense_pipe_bytes(int N, int Z, int word_bytes)
+{
+    const size_t state = ((size_t)2 * N * word_bytes + 15) & ~(size_t)15;
+    const size_t copies = word_bytes == 8 ? 2 : 1;
+    // + the spin order and the predecessor couplings, [N] each
+    return state + 2 * 3 * 64 * sizeof(double) + 3 * copies * (size_t)dense_pipe_slots(Z) * 16 + (size_t)N * 12;
+}
+
+template <typename WT>
+__device__ void dense_pipe_replica(const ColArgs<WT> &a, int r, unsigned char *base)
+{
+    const int N = a.N, W = a.W, Z = a.Z, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    WT *st = reinterpret_cast<WT *>(base);
+    double *ps = reinterpret_cast<double *>(base + (((size_t)2 * N * sizeof(WT) + 15) & ~(size_t)15));  // [2][3][64]
+    const int C = dense_pipe_slots(Z);
+    constexpr int kCopies = sizeof(WT) == 8 ? 2 : 1;
+    const int sw = wv - 1;  // summing wavefront 0..2 (wavefront 0 decides)
+    uint4 *rec = reinterpret_cast<uint4 *>(ps + 2 * 3 * 64) + (size_t)(sw < 0 ? 0 : sw) * kCopies * C;
+    double *pj = reinterpret_cast<double *>(reinterpret_cast<uint4 *>(ps + 2 * 3 * 64) + (size_t)3 * kCopies * C);  // [N]
+    int *ord = reinterpret_cast<int *>(pj + N);  // [N]
+    const int parts = (W <= 32 && 64 % W == 0) ? 64 / W : 1;
+    const int lb = parts > 1 ? lane % W : (lane < W ? lane : 0);
+    const int grp = parts > 1 ? lane / W : 0;
+    const int gp = sw * parts + grp, stride = 3 * parts;
+    const int Cg = (Z + stride - 1) / stride;  // records per lane group
+    const uint4 *rb = rec + (sizeof(WT) == 8 && lb >= 32 ? C : 0) + grp * Cg;
+    const bool active = lane < W;  // in wavefront 0: the lanes that decide slice lb
+    const WT one = WT(1);
+    const uint32_t rep = a.replica_base + (uint32_t)r;
+    WT *gst = a.packed + (size_t)r * 2 * N;
+    for (int q = tid; q < 2 * N; q += 256) st[q] = gst[q];
+    for (int q = tid; q < N; q += 256) {
+        ord[q] = a.order[q];
+        pj[q] = a.prevj[q];
+    }
+    // slot t of summing wavefront sw is row entry slot_e(t), stored as record
+    // (t % parts) Cg + t / parts: lane group grp reads records grp Cg + u,
+    // u = 0, 1, ... = entries gp + u stride
+    auto slot_e = [&](int t) { return (t / parts) * stride + sw * parts + t % parts; };
+    auto stage = [&](int t, int2 rw, int i, int ex, const WT *mine) __attribute__((always_inline)) {
+        const bool x = rw.x == ex;
+        const double J = x ? 0.0 : (double)__int_as_float(rw.y);
+        const WT w = (x || rw.x == i) ? WT(0) : mine[rw.x];
+        const int pos = (t % parts) * Cg + t / parts;
+        const uint32_t jlo = (uint32_t)__double2loint(J), jhi = (uint32_t)__double2hiint(J);
+        rec[pos] = make_uint4(jlo, jhi, (uint32_t)w, 0u);
+        if constexpr (sizeof(WT) == 8) rec[C + pos] = make_uint4(jlo, jhi, (uint32_t)((uint64_t)w >> 32), 0u);
+    };
+    constexpr int kPre = 2;  // slots lane + 64 k, read one spin ahead
+    int2 pre[kPre];
+    auto fetch_row = [&](int i) {
+        const int2 *row = a.rows + (size_t)i * Z;
+#pragma unroll
+        for (int k = 0; k < kPre; ++k) pre[k] = row[min(slot_e(lane + 64 * k), Z - 1)];
+    };
+    const uint32_t shl = 31u - (uint32_t)(lb & 31);
+    auto term = [&](double J, WT w) __attribute__((always_inline)) {
+        const uint32_t half = sizeof(WT) == 8 && lb >= 32 ? (uint32_t)((uint64_t)w >> 32) : (uint32_t)w;
+        return __longlong_as_double(__double_as_longlong(J) ^ ((long long)((half << shl) & 0x80000000u) << 32));
+    };
+    // a record's coupling with the sign of this lane's spin applied
+    auto rterm = [&](uint4 r) __attribute__((always_inline)) {
+        return __hiloint2double((int)(r.y ^ ((r.z << shl) & 0x80000000u)), (int)r.x);
+    };
+    const long long K = (long long)a.S * a.M * 2 * N;
+    int q = 0, tau = 0, m = 0, s = 0;   // step k
+    int pq = 0, ptau = 0, pm = 0, pso = 0;  // step k - 1
+    int iprev = -1;                      // spin of step k - 1 in the same half-sweep, else -1
+    WT lastw = 0;                        // wavefront 0: new word of the spin it decided last
+    if (sw >= 0) fetch_row(__builtin_amdgcn_readfirstlane(a.order[0]));
+    lds_barrier();
+    float jp = 0.0f;  // wavefront 0: J_perp and temperature of field step pso
+    double Ts = 0.0;
+    int js = -1;
+    for (long long k = 0; k <= K; ++k) {
+        if (wv == 0) {
+            if (k > 0) {
+                const int i = __builtin_amdgcn_readfirstlane(ord[pq]);
+                WT *mine = st + ptau * N;
+                const WT *other = st + (1 - ptau) * N;
+                const double *pp = ps + ((k - 1) & 1) * 3 * 64;
+                double h = (pp[lb] + pp[64 + lb]) + pp[128 + lb];
+                if (pq > 0) h += term(pj[pq], lastw);
+                if (pso != js) {
+                    js = pso;
+                    jp = a.jperp[pso];
+                    Ts = (double)a.temps[pso];
+                }
+                const uint32_t g = a.sweep_base + (uint32_t)(pso * a.M + pm);
+                const WT S_ = mine[i], O = other[i];
+                const int ml = ptau == 0 ? (lb == 0 ? W - 1 : lb - 1) : lb;
+                const int mr = ptau == 0 ? lb : (lb == W - 1 ? 0 : lb + 1);
+                const double si = ((S_ >> lb) & one) ? -1.0 : 1.0;
+                const double ts = (((O >> ml) & one) ? -1.0 : 1.0) + (((O >> mr) & one) ? -1.0 : 1.0);
+                const double ed = -2.0 * si * (h + (double)jp * ts);
+                unsigned int thr = 0;
+                const int cls = piqmc_fixed_class(ed, Ts, &thr);
+                bool fl = active && cls == 1;
+                if (__ballot(active && cls == 2)) {
+                    constexpr int kPer = sizeof(WT) == 4 ? 4 : 2;
+                    bool und = active && cls == 2, lt = false;
+                    for (int qd = 0; qd < 32 / kPer && __ballot(und); ++qd) {
+                        uint32_t out[4];
+                        philox4x32_10(g, (uint32_t)i, rep, (uint32_t(ptau) << 24) | (uint32_t)qd, a.key0, a.key1, out);
+#pragma unroll
+                        for (int t = 0; t < kPer; ++t) {
+                            const uint32_t ub = (uint32_t)((rword<WT>(out, t) >> lb) & one);
+                            const uint32_t tb = (thr >> (31 - (qd * kPer + t))) & 1u;
+                            if (und && ub != tb) {
+                                lt = ub < tb;
+                                und = false;
+                            }
+                        }
+                    }
+                    if (active && cls == 2) fl = lt;
+                }
+                lastw = S_ ^ (WT)__ballot(fl);
+                if (lane == 0) mine[i] = lastw;
+                if (a.etrace && pq == N - 1 && ptau == 1 && pm == a.M - 1) {
+                    double acc = 0.0;
+                    for (int i2 = lane; i2 < N; i2 += 64) {
+                        const WT a0 = st[i2], a1 = st[N + i2];
+                        const int2 *row = a.rows + (size_t)i2 * a.Z;
+                        for (int e = 0; e < a.Z; ++e) {
+                            const int2 rw = row[e];
+                            const double J = (double)__int_as_float(rw.y);
+                            if (J == 0.0) continue;
+                            if (rw.x == i2) acc -= J * (double)(a.P - 2 * (popc(a0) + popc(a1)));
+                            else acc -= 0.5 * J * (double)(a.P - 2 * (popc(a0 ^ st[rw.x]) + popc(a1 ^ st[N + rw.x])));
+                        }
+                    }
+                    acc = wave_sum(acc);
+                    if (lane == 0) a.etrace[(size_t)r * a.S + pso] = acc / (double)a.P;
+                }
+            }
+        } else if (k < K) {
+            const int i = __builtin_amdgcn_readfirstlane(ord[q]);
+            const WT *mine = st + tau * N;
+            // stage this wavefront's entries of row i; the entries for iprev
+            // (decided concurrently by wavefront 0) count zero here
+#pragma unroll
+            for (int kk = 0; kk < kPre; ++kk) {
+                const int t = lane + 64 * kk;
+                if (slot_e(t) < Z) stage(t, pre[kk], i, iprev, mine);
+            }
+            for (int t = lane + 64 * kPre; slot_e(t) < Z; t += 64)
+                stage(t, a.rows[(size_t)i * Z + slot_e(t)], i, iprev, mine);
+            if (k + 1 < K) fetch_row(__builtin_amdgcn_readfirstlane(ord[q + 1 < N ? q + 1 : 0]));
+            // the slots were written by this wavefront: LDS keeps its order
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            constexpr int G = 8;
+            double hk[4] = {0.0, 0.0, 0.0, 0.0};
+            const int T = Z > gp ? (Z - gp + stride - 1) / stride : 0;
+            int u = 0;
+            if (T >= G) {
+                // groups of G records, the next group read while this one is summed
+                uint4 rr[G];
+#pragma unroll
+                for (int kk = 0; kk < G; ++kk) rr[kk] = rb[kk];
+                for (; u + G <= T; u += G) {
+                    const int un = min(u + G, T - G);
+                    uint4 rn[G];
+#pragma unroll
+                    for (int kk = 0; kk < G; ++kk) rn[kk] = rb[un + kk];
+#pragma unroll
+                    for (int kk = 0; kk < G; ++kk) hk[kk & 3] += rterm(rr[kk]);
+#pragma unroll
+                    for (int kk = 0; kk < G; ++kk) rr[kk] = rn[kk];
+                }
+            }
+            for (; u < T; ++u) hk[0] += rterm(rb[u]);
+            double hp = (hk[0] + hk[1]) + (hk[2] + hk[3]);
+            for (int sh = W; sh < 64 && parts > 1; sh <<= 1) hp += __shfl_xor(hp, sh, 64);
+            if (lane < W) ps[(k & 1) * 3 * 64 + sw * 64 + lane] = hp;
+        }
+        // the wavefronts exchange data through LDS only, so the row prefetch
+        // stays in flight across the barrier
+        lds_barrier();
+        pq = q; ptau = tau; pm = m; pso = s;
+        iprev = __builtin_amdgcn_readfirstlane(ord[q]);
+        if (++q == N) {
+            q = 0;
+            iprev = -1;
+            if (++tau == 2) {
+                tau = 0;
+                if (++m == a.M) {
+                    m = 0;
+                    ++s;
+                }
+            }
+        }
+    }
+    for (int q2 = tid; q2 < 2 * N; q2 += 256) gst[q2] = st[q2];
+}
+
+template <typename WT>
+__global__ void __launch_bounds__(256) dense_pipe_kernel(ColArgs<WT> a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    dense_pipe_replica<WT>(a, blockIdx.x, smem);
+}
+
+inline bool dense_pipe_enabled()
+{
+    const char *e = getenv("PIQMC_DENSE_PIPE");
+    return !(e && e[0] == '0');
 }
 
 // One replica per wavefront, four wavefronts per workgroup, no barriers.
@@ -1459,10 +1694,12 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         if (block > 1024 || (block & 63)) return piqmc::set_error(PIQMC_EINVAL, "block must be a multiple of 64, <= 1024");
         const int wpb = block / 64;
         const size_t lds = dense_wave_bytes(N, pb->Z, sizeof(WT)) * wpb;
-        const size_t lwg = dense_wg_bytes(N, pb->Z, sizeof(WT));
         // a workgroup per replica while replicas are few (1.2-1.9x up to 1024
         // 256-spin replicas); from 4096 on, a wavefront each is 1.3x faster
-        const bool wg = pb->exact_rows && dense_wg_enabled() && lwg <= 160 * 1024 && R < 2048;
+        const bool wg0 = pb->exact_rows && dense_wg_enabled() && R < 2048;
+        const bool pipe = wg0 && dense_pipe_enabled() && dense_pipe_bytes(N, pb->Z, sizeof(WT)) <= 160 * 1024;
+        const size_t lwg = pipe ? dense_pipe_bytes(N, pb->Z, sizeof(WT)) : dense_wg_bytes(N, pb->Z, sizeof(WT));
+        const bool wg = wg0 && lwg <= 160 * 1024;
         if (!wg && lds > 160 * 1024) return piqmc::set_error(PIQMC_EINVAL, "dense sweep: state exceeds LDS");
         float *d_jp = nullptr;  // [S] J_perp, then [S] temperatures
         WT *d_packed = nullptr;
@@ -1476,11 +1713,15 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         a.ncolors = pb->ncolors; a.temps = d_jp + S;
         a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
         a.replica_base = rbase; a.sweep_base = sbase;
-        a.jperp = d_jp; a.rows = pb->d_rows; a.order = pb->d_order;
+        a.jperp = d_jp; a.rows = pb->d_rows; a.order = pb->d_order; a.prevj = pb->d_prevj;
         a.packed = d_packed; a.etrace = etrace;
         piqmc::timing_begin(st);
         const int W = P / 2;
-        if (wg) {
+        if (wg && pipe) {
+            // one replica per workgroup, one spin decided while the next is summed
+            (void)hipFuncSetAttribute((const void *)dense_pipe_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lwg);
+            dense_pipe_kernel<WT><<<R, 256, lwg, st>>>(a);
+        } else if (wg) {
             // one replica per workgroup (the field split over four wavefronts)
             (void)hipFuncSetAttribute((const void *)dense_wg_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lwg);
             dense_wg_kernel<WT><<<R, 256, lwg, st>>>(a);
@@ -1542,7 +1783,7 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
     a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
     a.replica_base = rbase; a.sweep_base = sbase;
     a.jperp = d_jp; a.rows = pb->d_rows; a.order = pb->d_order; a.color_off = d_coff;
-    a.cnb = pb->d_cnb; a.cinfo = pb->d_cinfo; a.thr = d_thr; a.ztab = d_zt; a.hq = pb->d_hq;
+    a.cnb = pb->d_cnb; a.cinfo = pb->d_cinfo; a.thr = d_thr; a.ztab = d_zt; a.hq = pb->d_hq; a.prevj = pb->d_prevj;
     a.packed = d_packed; a.etrace = etrace;
     const int grid = (R + rpw - 1) / rpw;
     piqmc::timing_begin(st);

@@ -258,9 +258,20 @@ static int upload_coloring(piqmc_problem &p)
     if (p.d_cnb) (void)hipFree(p.d_cnb);
     if (p.d_cinfo) (void)hipFree(p.d_cinfo);
     if (p.d_hq) (void)hipFree(p.d_hq);
-    p.d_order = nullptr; p.d_cnb = nullptr; p.d_cinfo = nullptr; p.d_hq = nullptr;
+    if (p.d_prevj) (void)hipFree(p.d_prevj);
+    p.d_order = nullptr; p.d_cnb = nullptr; p.d_cinfo = nullptr; p.d_hq = nullptr; p.d_prevj = nullptr;
     PIQMC_HIP(hipMalloc(&p.d_order, sizeof(int32_t) * N));
     PIQMC_HIP(hipMemcpy(p.d_order, order.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
+    // coupling of spin order[q] to its predecessor order[q-1], summed over
+    // the row's entries for that neighbour (0 for q = 0)
+    std::vector<double> prevj(N, 0.0);
+    for (int q = 1; q < N; ++q) {
+        const int i = order[q], pj = order[q - 1];
+        for (int e = 0; e < p.Z; ++e)
+            if (p.idx[(size_t)i * p.Z + e] == pj) prevj[q] += (double)p.J[(size_t)i * p.Z + e];
+    }
+    PIQMC_HIP(hipMalloc(&p.d_prevj, sizeof(double) * N));
+    PIQMC_HIP(hipMemcpy(p.d_prevj, prevj.data(), sizeof(double) * N, hipMemcpyHostToDevice));
     if (p.pmj || p.pmjf) {
         const int zm = std::max(p.zmax, 1);
         std::vector<int32_t> cnb((size_t)zm * N, 0);
@@ -444,6 +455,7 @@ extern "C" int piqmc_problem_destroy(piqmc_problem *p)
     if (p->d_cnb) (void)hipFree(p->d_cnb);
     if (p->d_cinfo) (void)hipFree(p->d_cinfo);
     if (p->d_hq) (void)hipFree(p->d_hq);
+    if (p->d_prevj) (void)hipFree(p->d_prevj);
     delete p;
     return PIQMC_OK;
 }

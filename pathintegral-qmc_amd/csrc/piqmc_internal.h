@@ -60,6 +60,9 @@ struct piqmc_problem {
     // with E - L <= 52): the lane-per-slice sweep may then split and
     // reassociate the sum without changing a bit.
     int exact_rows = 0;
+    // [N] sum of the couplings of spin order[q] to spin order[q-1] (0 at q = 0):
+    // the pipelined dense sweep adds that neighbour's term after its decision
+    double *d_prevj = nullptr;
 };
 
 // ------------------------------------------------------------------ device helpers
