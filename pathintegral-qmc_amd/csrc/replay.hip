@@ -137,6 +137,13 @@ __global__ void replay_kernel(ReplayArgs a)
                 if (own) load(cur, k);
                 float e = 0.0f;
                 int inext = STAGED ? (int)lperm[lane] : (int)pg[0];
+                // the first four row entries of the next attempt's spin, read
+                // one attempt ahead (the staged rows are read-only)
+                int2 rwn[4];
+                if constexpr (STAGED && !F64J) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) rwn[q] = rows[__umul24((uint32_t)inext, (uint32_t)Zr) + q];
+                }
                 for (int t = 0; t < N; ++t) {
                     const int i = inext;  // the next position's index is read one attempt ahead
                     const int tn = min(t + 1, N - 1);
@@ -154,7 +161,7 @@ __global__ void replay_kernel(ReplayArgs a)
                     int2 rw[4];
                     if constexpr (STAGED && !F64J) {
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) rw[q] = row[q];
+                        for (int q = 0; q < 4; ++q) rw[q] = rwn[q];
                     }
                     __builtin_amdgcn_sched_barrier(0);
                     const uint32_t bsh = (uint32_t)i & 31u;
@@ -233,6 +240,10 @@ __global__ void replay_kernel(ReplayArgs a)
                             flip = ex > u;
                             if (ex > 0.0 && ex < INFINITY && fabs(ex - u) <= ex * 0x1p-50) ++nt;
                         }
+                    }
+                    if constexpr (STAGED && !F64J) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) rwn[q] = rows[__umul24((uint32_t)inext, (uint32_t)Zr) + q];
                     }
                     if (flip) *wself = ws ^ bi;
                 }
