@@ -1488,7 +1488,7 @@ __global__ void __launch_bounds__(256) dense_pipe_kernel(ColArgs<WT> a)
 
 inline bool dense_pipe_enabled()
 {
-    const char *e = getenv("PIQMC_DENSE_PIPE");
+    const char *e = std::getenv("PIQMC_DENSE_PIPE");
     return !(e && e[0] == '0');
 }
 
