@@ -1281,13 +1281,15 @@ __device__ __forceinline__ void lds_barrier()
 // A staged entry is one 16-byte record (coupling, spin word) that a lane reads
 // with one LDS instruction; with 64-bit words a second copy carries the high
 // half for lanes 32-63.  Each lane group's records are contiguous.
-__host__ __device__ inline int dense_pipe_slots(int Z) { return (Z + 2) / 3 + 64; }
-__host__ __device__ inline size_t dense_pipe_bytes(int N, int Z, int word_bytes)
+// nsw: summing wavefronts (the workgroup's wavefronts but one)
+__host__ __device__ inline int dense_pipe_slots(int Z, int nsw) { return (Z + nsw - 1) / nsw + 64; }
+__host__ __device__ inline size_t dense_pipe_bytes(int N, int Z, int word_bytes, int nsw = 3)
 {
     const size_t state = ((size_t)2 * N * word_bytes + 15) & ~(size_t)15;
     const size_t copies = word_bytes == 8 ? 2 : 1;
     // + the spin order and the predecessor couplings, [N] each
-    return state + 2 * 3 * 64 * sizeof(double) + 3 * copies * (size_t)dense_pipe_slots(Z) * 16 + (size_t)N * 12;
+    return state + 2 * (size_t)nsw * 64 * sizeof(double) + (size_t)nsw * copies * dense_pipe_slots(Z, nsw) * 16 +
+           (size_t)N * 12;
 }
 
 template <typename WT>
@@ -1295,25 +1297,26 @@ __device__ void dense_pipe_replica(const ColArgs<WT> &a, int r, unsigned char *b
 {
     const int N = a.N, W = a.W, Z = a.Z, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     WT *st = reinterpret_cast<WT *>(base);
-    double *ps = reinterpret_cast<double *>(base + (((size_t)2 * N * sizeof(WT) + 15) & ~(size_t)15));  // [2][3][64]
-    const int C = dense_pipe_slots(Z);
+    double *ps = reinterpret_cast<double *>(base + (((size_t)2 * N * sizeof(WT) + 15) & ~(size_t)15));  // [2][nsw][64]
+    const int nsw = (int)(blockDim.x >> 6) - 1, nth = (int)blockDim.x;
+    const int C = dense_pipe_slots(Z, nsw);
     constexpr int kCopies = sizeof(WT) == 8 ? 2 : 1;
     const int sw = wv - 1;  // summing wavefront 0..2 (wavefront 0 decides)
-    uint4 *rec = reinterpret_cast<uint4 *>(ps + 2 * 3 * 64) + (size_t)(sw < 0 ? 0 : sw) * kCopies * C;
-    double *pj = reinterpret_cast<double *>(reinterpret_cast<uint4 *>(ps + 2 * 3 * 64) + (size_t)3 * kCopies * C);  // [N]
+    uint4 *rec = reinterpret_cast<uint4 *>(ps + 2 * nsw * 64) + (size_t)(sw < 0 ? 0 : sw) * kCopies * C;
+    double *pj = reinterpret_cast<double *>(reinterpret_cast<uint4 *>(ps + 2 * nsw * 64) + (size_t)nsw * kCopies * C);  // [N]
     int *ord = reinterpret_cast<int *>(pj + N);  // [N]
     const int parts = (W <= 32 && 64 % W == 0) ? 64 / W : 1;
     const int lb = parts > 1 ? lane % W : (lane < W ? lane : 0);
     const int grp = parts > 1 ? lane / W : 0;
-    const int gp = sw * parts + grp, stride = 3 * parts;
+    const int gp = sw * parts + grp, stride = nsw * parts;
     const int Cg = (Z + stride - 1) / stride;  // records per lane group
     const uint4 *rb = rec + (sizeof(WT) == 8 && lb >= 32 ? C : 0) + grp * Cg;
     const bool active = lane < W;  // in wavefront 0: the lanes that decide slice lb
     const WT one = WT(1);
     const uint32_t rep = a.replica_base + (uint32_t)r;
     WT *gst = a.packed + (size_t)r * 2 * N;
-    for (int q = tid; q < 2 * N; q += 256) st[q] = gst[q];
-    for (int q = tid; q < N; q += 256) {
+    for (int q = tid; q < 2 * N; q += nth) st[q] = gst[q];
+    for (int q = tid; q < N; q += nth) {
         ord[q] = a.order[q];
         pj[q] = a.prevj[q];
     }
@@ -1362,8 +1365,9 @@ __device__ void dense_pipe_replica(const ColArgs<WT> &a, int r, unsigned char *b
                 const int i = __builtin_amdgcn_readfirstlane(ord[pq]);
                 WT *mine = st + ptau * N;
                 const WT *other = st + (1 - ptau) * N;
-                const double *pp = ps + ((k - 1) & 1) * 3 * 64;
-                double h = (pp[lb] + pp[64 + lb]) + pp[128 + lb];
+                const double *pp = ps + ((k - 1) & 1) * nsw * 64;
+                double h = pp[lb];
+                for (int w = 1; w < nsw; ++w) h += pp[w * 64 + lb];
                 if (pq > 0) h += term(pj[pq], lastw);
                 if (pso != js) {
                     js = pso;
@@ -1457,7 +1461,7 @@ __device__ void dense_pipe_replica(const ColArgs<WT> &a, int r, unsigned char *b
             for (; u < T; ++u) hk[0] += rterm(rb[u]);
             double hp = (hk[0] + hk[1]) + (hk[2] + hk[3]);
             for (int sh = W; sh < 64 && parts > 1; sh <<= 1) hp += __shfl_xor(hp, sh, 64);
-            if (lane < W) ps[(k & 1) * 3 * 64 + sw * 64 + lane] = hp;
+            if (lane < W) ps[(k & 1) * nsw * 64 + sw * 64 + lane] = hp;
         }
         // the wavefronts exchange data through LDS only, so the row prefetch
         // stays in flight across the barrier
@@ -1476,14 +1480,27 @@ __device__ void dense_pipe_replica(const ColArgs<WT> &a, int r, unsigned char *b
             }
         }
     }
-    for (int q2 = tid; q2 < 2 * N; q2 += 256) gst[q2] = st[q2];
+    for (int q2 = tid; q2 < 2 * N; q2 += nth) gst[q2] = st[q2];
 }
 
 template <typename WT>
-__global__ void __launch_bounds__(256) dense_pipe_kernel(ColArgs<WT> a)
+__global__ void __launch_bounds__(512) dense_pipe_kernel(ColArgs<WT> a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     dense_pipe_replica<WT>(a, blockIdx.x, smem);
+}
+
+// Wavefronts per pipelined workgroup: eight (seven summing) when a lane's
+// share of the row with three summing wavefronts is 64 terms or more and the
+// replicas are few enough for one workgroup per CU (256-spin SK at P = 128,
+// 4-256 replicas: 58 -> 46 ms); four otherwise (eight were 5-70 % slower at
+// P <= 64 or 1024 replicas).  PIQMC_DENSE_PIPE_WAVES = 4 or 8 forces a count.
+inline int dense_pipe_waves(int Z, int W, int R)
+{
+    const char *e = std::getenv("PIQMC_DENSE_PIPE_WAVES");
+    if (e && (e[0] == '4' || e[0] == '8')) return e[0] - '0';
+    const int parts = (W <= 32 && 64 % W == 0) ? 64 / W : 1;
+    return ((Z + 3 * parts - 1) / (3 * parts) >= 64 && R <= 256) ? 8 : 4;
 }
 
 inline bool dense_pipe_enabled()
@@ -1697,8 +1714,9 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         // a workgroup per replica while replicas are few (1.2-1.9x up to 1024
         // 256-spin replicas); from 4096 on, a wavefront each is 1.3x faster
         const bool wg0 = pb->exact_rows && dense_wg_enabled() && R < 2048;
-        const bool pipe = wg0 && dense_pipe_enabled() && dense_pipe_bytes(N, pb->Z, sizeof(WT)) <= 160 * 1024;
-        const size_t lwg = pipe ? dense_pipe_bytes(N, pb->Z, sizeof(WT)) : dense_wg_bytes(N, pb->Z, sizeof(WT));
+        const int pw = dense_pipe_waves(pb->Z, P / 2, R);
+        const bool pipe = wg0 && dense_pipe_enabled() && dense_pipe_bytes(N, pb->Z, sizeof(WT), pw - 1) <= 160 * 1024;
+        const size_t lwg = pipe ? dense_pipe_bytes(N, pb->Z, sizeof(WT), pw - 1) : dense_wg_bytes(N, pb->Z, sizeof(WT));
         const bool wg = wg0 && lwg <= 160 * 1024;
         if (!wg && lds > 160 * 1024) return piqmc::set_error(PIQMC_EINVAL, "dense sweep: state exceeds LDS");
         float *d_jp = nullptr;  // [S] J_perp, then [S] temperatures
@@ -1720,7 +1738,7 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         if (wg && pipe) {
             // one replica per workgroup, one spin decided while the next is summed
             (void)hipFuncSetAttribute((const void *)dense_pipe_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lwg);
-            dense_pipe_kernel<WT><<<R, 256, lwg, st>>>(a);
+            dense_pipe_kernel<WT><<<R, 64 * pw, lwg, st>>>(a);
         } else if (wg) {
             // one replica per workgroup (the field split over four wavefronts)
             (void)hipFuncSetAttribute((const void *)dense_wg_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lwg);

@@ -190,20 +190,22 @@ def test_colored_statistics_vs_sequential_oracle(gpu):
     assert abs(pg - ps) <= 3 * np.sqrt(max(pp * (1 - pp), 1e-12) * 2 / nseed) + 1e-12, (pg, ps)
 
 
-@pytest.mark.parametrize("wg", ["pipe", "1", "0"])
+@pytest.mark.parametrize("wg", ["pipe", "pipe4", "pipe8", "1", "0"])
 @pytest.mark.parametrize("tiny", [False, True])
 @pytest.mark.parametrize("N,P", [(64, 16), (100, 32), (256, 128), (40, 20), (48, 8), (64, 2), (80, 64)])
 def test_dense_path_sk_bit_exact(gpu, monkeypatch, N, P, tiny, wg):
     # Sherrington-Kirkpatrick couplings J_ij ~ N(0, 1/N): one color per spin,
     # the dense paths, against the CPU twin.  Such rows sum exactly in double:
     # one replica per workgroup, either pipelined (wavefront 0 decides a spin
-    # while three wavefronts sum the next one's field) or with the field split
-    # over four wavefronts (PIQMC_DENSE_PIPE=0), or (PIQMC_DENSE_WG=0) one
+    # while the other three or seven sum the next one's field; automatic or
+    # forced count) or with the field split over four wavefronts
+    # (PIQMC_DENSE_PIPE=0), or (PIQMC_DENSE_WG=0) one
     # replica per wavefront with the field split over 64/(P/2) lanes when P/2
     # divides 16.  One coupling of 1e-12 makes a row inexact and keeps the
     # row-order sum of one lane per slice.
     monkeypatch.setenv("PIQMC_DENSE_WG", "0" if wg == "0" else "1")
-    monkeypatch.setenv("PIQMC_DENSE_PIPE", "1" if wg == "pipe" else "0")
+    monkeypatch.setenv("PIQMC_DENSE_PIPE", "1" if wg.startswith("pipe") else "0")
+    monkeypatch.setenv("PIQMC_DENSE_PIPE_WAVES", {"pipe4": "4", "pipe8": "8"}.get(wg, "auto"))
     from piqmc import instances
     n, r, c, v = instances.sk(N, np.random.RandomState(N))
     if tiny:
@@ -218,13 +220,16 @@ def test_dense_path_sk_bit_exact(gpu, monkeypatch, N, P, tiny, wg):
     np.testing.assert_allclose(etg, etr, rtol=1e-12, atol=1e-10)
 
 
-@pytest.mark.parametrize("N,P,M", [(96, 48, 2), (33, 6, 3), (256, 32, 2)])
-def test_dense_pipe_sweeps_and_bases(gpu, monkeypatch, N, P, M):
-    # the pipelined workgroup path over several sweeps per field step, with
-    # replica and sweep bases: the spin decided in one step and the field
-    # summed in the next cross sweep and half-sweep boundaries
+@pytest.mark.parametrize("waves", ["4", "8"])
+@pytest.mark.parametrize("N,P,M", [(96, 48, 2), (33, 6, 3), (256, 32, 2), (200, 128, 2)])
+def test_dense_pipe_sweeps_and_bases(gpu, monkeypatch, N, P, M, waves):
+    # the pipelined workgroup path (three or seven summing wavefronts) over
+    # several sweeps per field step, with replica and sweep bases: the spin
+    # decided in one step and the field summed in the next cross sweep and
+    # half-sweep boundaries
     monkeypatch.setenv("PIQMC_DENSE_WG", "1")
     monkeypatch.setenv("PIQMC_DENSE_PIPE", "1")
+    monkeypatch.setenv("PIQMC_DENSE_PIPE_WAVES", waves)
     from piqmc import instances
     n, r, c, v = instances.sk(N, np.random.RandomState(N + 1))
     nbs = instances.neighbors(n, r, c, v)
