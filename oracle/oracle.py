@@ -59,6 +59,9 @@ def lib():
         L.orc_qa_colored.argtypes = [f64p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
                                      i8p, f64p, C.c_int, i32p, C.c_int, C.c_uint64,
                                      C.c_uint32, C.c_uint32, C.c_void_p]
+        L.orc_qa_colored_stats.argtypes = [f64p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
+                                           i8p, f64p, C.c_int, i32p, C.c_int, C.c_uint64,
+                                           C.c_uint32, np.ctypeslib.ndpointer(np.uint64, flags="C")]
         L.orc_sa_colored.argtypes = [f64p, C.c_int, C.c_int, C.c_int, C.c_int, i8p, f64p,
                                      C.c_int, i32p, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32]
         L.orc_sa_multispin.argtypes = [np.ctypeslib.ndpointer(np.float32, flags="C"), C.c_int,
@@ -70,7 +73,7 @@ def lib():
         L.orc_energy_coo.restype = C.c_double
         for name in ("orc_qa_compat", "orc_sa_compat", "orc_qa_dense", "orc_sa_dense",
                      "orc_qa_parallel",
-                     "orc_qa_fixed_seq", "orc_qa_colored", "orc_sa_colored"):
+                     "orc_qa_fixed_seq", "orc_qa_colored", "orc_qa_colored_stats", "orc_sa_colored"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -227,6 +230,22 @@ def qa_colored(sched, mcsteps, P, temp, spins, nbs, color, ncolors, seed, replic
     if rc:
         raise ValueError("orc_qa_colored rc=%d" % rc)
     return et
+
+
+def qa_colored_stats(sched, mcsteps, P, temp, spins, nbs, color, ncolors, seed, replica=0):
+    """qa_colored that also returns its work: [words, stochastic words,
+    stochastic attempts, Philox blocks of an early-exit comparator]."""
+    sched = np.ascontiguousarray(sched, np.float64)
+    nbs = np.ascontiguousarray(nbs, np.float64)
+    color = np.ascontiguousarray(color, np.int32)
+    assert spins.dtype == np.int8 and spins.flags.c_contiguous
+    st = np.zeros(4, np.uint64)
+    rc = lib().orc_qa_colored_stats(sched, len(sched), int(mcsteps), int(P), float(temp),
+                                    spins.shape[1], spins, nbs, nbs.shape[1], color, int(ncolors),
+                                    int(seed) & 0xFFFFFFFFFFFFFFFF, int(replica), st)
+    if rc:
+        raise ValueError("orc_qa_colored_stats rc=%d" % rc)
+    return st
 
 
 def sa_colored(sched, mcsteps, spins, nbs, color, ncolors, seed, replica_base=0, sweep_base=0):

@@ -16,8 +16,13 @@
  *   orc_rand_*        : libc srand()/rand() through ctypes, many seeds.
  *   orc_mt_*, shuffle : numpy.random.RandomState.permutation, chained.
  *   orc_philox4x32    : Random123 known-answer vectors (Salmon et al. SC'11).
- *   orc_qa_compat     : hand traces on N=2..4 (tests/golden), defect D1/D2
- *                       reductions, Boixo outcome test (testing/test_boixo.py:90-114).
+ *   orc_qa_compat     : hand traces on N=1..4: defects D1/D2, a field row, and
+ *                       non-dyadic couplings (0.1/0.2/-0.3) where the float32
+ *                       running ediff takes a different decision than a double
+ *                       sum or an uncast J would; Boixo outcome test
+ *                       (testing/test_boixo.py:90-114).
+ *   orc_jperp         : values fixed by the mathematics (tanh = e^-2 gives PT,
+ *                       small- and large-Gamma limits) and a float-P*T case.
  *   orc_sa_compat     : hand traces, Boixo SA outcome test (test_boixo.py:72-88).
  *   orc_qa_dense,     : equal to orc_qa_compat / orc_sa_compat (and so to their
  *   orc_sa_dense        pins) on matrices whose entries are float-exact, given
@@ -32,7 +37,10 @@
  *   orc_sa_multispin  : each of the 64 packed configurations equals an
  *                       independent +-1 Metropolis chain fed its own uniforms
  *                       (tests/test_oracle_multispin.py); hand trace.
- *   orc_qa_parallel   : timing twin only (the reference is racy), unpinned.
+ *   orc_qa_parallel   : hand traces at nthreads=1, where it is deterministic
+ *                       (ascending spins, per-spin double ediffs, D1, libc
+ *                       rand() only on the exp branch); with more threads it
+ *                       races like the reference and serves only as a timing twin.
  */
 #include <math.h>
 #include <stdint.h>
@@ -578,10 +586,15 @@ static void orc_sliced_words(uint32_t g, uint32_t i, uint32_t rep, uint32_t tau,
         }
     }
 }
+/* stats (optional, uint64[4]): parity words swept, words with at least one
+ * stochastic attempt, stochastic attempts, and the Philox blocks an
+ * MSB-first comparator with per-word early exit consumes (a bit is decided
+ * at the first uniform bit that differs from its threshold's; a block
+ * supplies 32 / words-per-call uniform bits of every slice of the word). */
 static int colored_core(const float *jps, const float *temps, int S, int M, int P, int N,
                         int8_t *sp, const double *nbs, int Z, const int32_t *color,
                         int ncolors, uint64_t seed, uint32_t replica, uint32_t sweep_base,
-                        double *etrace)
+                        double *etrace, uint64_t *stats)
 {
     if (P < 2 || (P & 1) || N < 1 || Z < 1 || ncolors < 1) return -1;
     for (int i = 0; i < N; ++i) {
@@ -603,7 +616,7 @@ static int colored_core(const float *jps, const float *temps, int S, int M, int 
                     for (int i = 0; i < N; ++i) {
                         if (color[i] != c) continue;
                         uint64_t R[32];
-                        int have_R = 0;
+                        int have_R = 0, nstoch = 0, need = 0;
                         for (int k = tau; k < P; k += 2) {
                             double ed = orc_fixed_ediff(sp, P, N, k, i, nbs, Z, jperp);
                             unsigned int thr = 0;
@@ -620,8 +633,25 @@ static int colored_core(const float *jps, const float *temps, int S, int M, int 
                                 for (int b = 0; b < 32; ++b)
                                     u |= (uint32_t)((R[b] >> mb) & 1u) << (31 - b);
                                 flip = u < thr;
+                                if (stats) {
+                                    /* uniform bits examined: up to the first that differs */
+                                    const uint32_t d = u ^ (uint32_t)thr;
+                                    int used = 32;
+                                    for (int b = 0; b < 32; ++b)
+                                        if ((d >> (31 - b)) & 1u) { used = b + 1; break; }
+                                    const int per = P / 2 > 32 ? 2 : 4;   /* uniform bits per block */
+                                    const int blocks = (used + per - 1) / per;
+                                    if (blocks > need) need = blocks;
+                                    ++nstoch;
+                                }
                             }
                             if (flip) sp[(size_t)k * N + i] = (int8_t)-sp[(size_t)k * N + i];
+                        }
+                        if (stats) {
+                            stats[0] += 1;
+                            stats[1] += nstoch > 0;
+                            stats[2] += (uint64_t)nstoch;
+                            stats[3] += (uint64_t)need;
                         }
                     }
                 }
@@ -645,7 +675,27 @@ int orc_qa_colored(const double *sched, int S, int M, int P, float temp, int N,
         jps[S + s] = temp;
     }
     int rc = colored_core(jps, jps + S, S, M, P, N, sp, nbs, Z, color, ncolors, seed, replica,
-                          sweep_base, etrace);
+                          sweep_base, etrace, NULL);
+    free(jps);
+    return rc;
+}
+
+/* orc_qa_colored that also counts its work (see colored_core): stats[0..3]
+ * = words, stochastic words, stochastic attempts, Philox blocks. */
+int orc_qa_colored_stats(const double *sched, int S, int M, int P, float temp, int N,
+                         int8_t *sp, const double *nbs, int Z, const int32_t *color,
+                         int ncolors, uint64_t seed, uint32_t replica, uint64_t *stats)
+{
+    if (P < 2) return -1;
+    float *jps = (float *)malloc(sizeof(float) * (S > 0 ? 2 * S : 1));
+    if (!jps) return -4;
+    for (int s = 0; s < S; ++s) {
+        jps[s] = orc_jperp(sched[s], P, temp);
+        jps[S + s] = temp;
+    }
+    for (int t = 0; t < 4; ++t) stats[t] = 0;
+    int rc = colored_core(jps, jps + S, S, M, P, N, sp, nbs, Z, color, ncolors, seed, replica, 0,
+                          NULL, stats);
     free(jps);
     return rc;
 }
@@ -669,7 +719,7 @@ int orc_sa_colored(const double *sched, int S, int M, int C, int N, int8_t *spin
     int rc = 0;
     for (int q = 0; q < C / P && !rc; ++q)
         rc = colored_core(jps, jps + S, S, M, P, N, spins + (size_t)q * P * N, nbs, Z, color,
-                          ncolors, seed, rbase + (uint32_t)q, sweep_base, NULL);
+                          ncolors, seed, rbase + (uint32_t)q, sweep_base, NULL, NULL);
     free(jps);
     return rc;
 }

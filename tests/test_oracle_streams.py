@@ -58,13 +58,26 @@ def test_philox_known_answers():
         ["0xd16cfe09", "0x94fdcceb", "0x5001e420", "0x24126ea1"]
 
 
-def test_jperp_closed_form():
-    # J_perp = -(P T / 2) log tanh(Gamma / (P T)), piqmc/qmc.pyx:95, float stores
-    for g, P, T in [(3.0, 20, 0.05), (1e-8, 64, 1 / 64), (0.5, 5, 0.01), (1.5, 10, 0.01)]:
-        Tf = np.float32(T)
-        expect = np.float32(-0.5 * P * float(Tf) * np.log(np.tanh(g / float(np.float32(P * Tf)))))
-        assert O.jperp(g, P, T) == expect
-        assert O.jperp(g, P, T) > 0
+def test_jperp_known_values():
+    # J_perp = -(P T / 2) log tanh(Gamma / (P T)), piqmc/qmc.pyx:95, stored as float.
+    # Points fixed by the mathematics, not by restating the expression:
+    atanh_e2 = np.arctanh(np.exp(-2.0))
+    for P, T in [(20, 0.05), (64, 1 / 64), (4, 0.25), (8, 0.125)]:
+        pt = P * T                                          # exactly 1 for these
+        # tanh(Gamma/PT) = e^-2  =>  J_perp = PT  (the -1/2 factor, the sign, natural log)
+        assert O.jperp(pt * atanh_e2, P, T) == np.float32(pt)
+        # Gamma << PT: tanh x = x (1 - x^2/3 + ..), so Gamma = PT e^-10 gives 5 PT
+        assert abs(O.jperp(pt * np.exp(-10.0), P, T) - 5 * pt) <= 4e-7 * pt
+        # Gamma >> PT: tanh rounds to 1.0, log to 0 -> exactly zero coupling
+        assert O.jperp(50.0 * pt, P, T) == 0.0
+        # positive and decreasing in Gamma (ferromagnetic between slices)
+        js = [O.jperp(g, P, T) for g in (1e-8, 0.01, 0.1, 1.0, 3.0)]
+        assert all(a > b > 0 for a, b in zip(js, js[1:]))
+    # P*T is a float product (qmc.c:2058-2075, __pyx_t_12 = slices * temp):
+    # float(3 * 0.1f) = 0.30000001192.. where the double product is 0.30000000447..;
+    # at Gamma = 0.01 that moves J_perp by one float ulp
+    assert O.jperp(0.01, 3, 0.1) == np.float32(0.5102351903915405)
+    assert O.jperp(0.01, 3, 0.1) != np.float32(0.5102351307868958)   # double P*T
 
 
 def test_fexp64_accuracy():

@@ -57,6 +57,99 @@ def test_trace_field_term():
     assert confs.tolist() == [[1.0, 1.0]] and d == 0
 
 
+def _star_nbs():
+    # spin 0 coupled to spins 1, 2, 3 with J = 0.1, 0.2, -0.3 (float32 values
+    # 0.1000000015, 0.2000000030, -0.3000000119); rows padded with [0, 0]
+    return np.array([[[1, 0.1], [2, 0.2], [3, -0.3]],
+                     [[0, 0.1], [0, 0.0], [0, 0.0]],
+                     [[0, 0.2], [0, 0.0], [0, 0.0]],
+                     [[0, -0.3], [0, 0.0], [0, 0.0]]])
+
+
+def test_trace_float32_running_ediff_decides():
+    # Non-dyadic couplings where the reference's float32 running ediff
+    # (piqmc/qmc.pyx:87,107-113; each term computed in double and stored to
+    # float, qmc.c:2192-2226) takes a different decision than a double sum.
+    # P=2, J_perp=0 (Gamma=1e6), T=1e-10, perm [0,1,2,3]; spin 0 = -1, others
+    # +1 in both slices; srand(1).  Per slice (the slices are independent):
+    #  spin0 (c=-1): e = f(0.2f) = 0.2000000030; e = f(0.2000000030 + 0.4000000060)
+    #        = f(0.6000000089) = 0.6000000238; e = f(0.6000000238 - 0.6000000238)
+    #        = 0 -> draw, exp(0) = 1 > u -> flip to +1.  (A double sum gives
+    #        -1.49e-8 and keeps -1; dropping the (float) cast of J gives +2.4e-8
+    #        and flips without a draw.)
+    #  spin1: e = f(0 - 0.2000000030) -> draw, exp(-2e9) = 0 -> stay
+    #  spin2: e = f(-0.2000000030 - 0.4000000060) = -0.6000000238 -> draw, stay
+    #  spin3: e = f(-0.6000000238 + 0.6000000238) = 0 (running sum, D2) -> draw,
+    #         flip to -1.
+    # => both slices [+1, +1, +1, -1], 8 draws.
+    confs = np.array([[-1.0, -1.0], [1.0, 1.0], [1.0, 1.0], [1.0, 1.0]])
+    rs = O.Type3(1)
+    d = O.qa_compat(np.array([1e6]), 1, 2, 1e-10, 4, confs, _star_nbs(), np.array([[0, 1, 2, 3]]), rs)
+    assert confs.tolist() == [[1.0, 1.0], [1.0, 1.0], [1.0, 1.0], [-1.0, -1.0]]
+    assert d == 8
+    assert rs.next() == 596516649          # the 9th rand() after srand(1)
+
+
+def _libc_rand():
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    return libc
+
+
+def test_trace_parallel_variant_one_thread():
+    # QuantumAnneal_parallel (piqmc/qmc.pyx:320-357) at nthreads=1 visits spins
+    # in ascending order with one float64 ediffs[sidx] per spin, reset after
+    # each slice (so per spin), J as float, D1 Trotter slices, accept on > 0
+    # else exp(ediff/T) > rand()/RAND_MAX (qmc.c:4295-4345).  Same star as
+    # above, P=2, J_perp=0, T=1e-10, libc srand(1).  Per slice:
+    #  spin0 (c=-1): e = 0.2000000030 + 0.4000000060 - 0.6000000238 = -1.49e-8
+    #        (double) -> draw, exp(-149) ~ 0 -> stay -1
+    #  spin1 (c=+1, sees c0=-1): e = -2(0.1f)(-1) = +0.2 > 0 -> flip to -1
+    #  spin2: e = +0.4 > 0 -> flip to -1
+    #  spin3: e = -2(-0.3f)(-1) = -0.6 -> draw, stay +1
+    # => both slices [-1, -1, -1, +1], 4 draws; the next rand() is the 5th.
+    libc = _libc_rand()
+    confs = np.array([[-1.0, -1.0], [1.0, 1.0], [1.0, 1.0], [1.0, 1.0]])
+    libc.srand(1)
+    O.qa_parallel(np.array([1e6]), 1, 2, 1e-10, 4, confs, _star_nbs(), 1)
+    assert libc.rand() == 1957747793
+    assert confs.tolist() == [[-1.0, -1.0], [-1.0, -1.0], [-1.0, -1.0], [1.0, 1.0]]
+
+
+def test_trace_parallel_variant_trotter_D1():
+    # The parallel variant keeps defect D1 (tidx never set, piqmc/qmc.pyx:337-345):
+    # N=1 with a zero field row, P=4, T=0.25, Gamma=0.01 (J_perp ~ 2.30), start
+    # [+1,-1,+1,-1].  k=0: slices 3, 1 are -1 -> e = +4 Jp > 0 -> -1.  k=1: slices
+    # 3 and itself (-1) -> e = -4 Jp -> draw, exp(-36.8) -> stay.  k=2: +4 Jp ->
+    # -1.  k=3: itself and slice 1 -> -4 Jp -> draw -> stay.  2 draws.  With
+    # periodic neighbours k-1, k+1 slice 1 would see e = 0 at k=1 and flip.
+    libc = _libc_rand()
+    confs = np.array([[1.0, -1.0, 1.0, -1.0]])
+    libc.srand(1)
+    O.qa_parallel(np.array([0.01]), 1, 4, 0.25, 1, confs, np.array([[[0, 0.0]]]), 1)
+    assert libc.rand() == 1681692777
+    assert confs.tolist() == [[-1.0, -1.0, -1.0, -1.0]]
+
+
+def test_trace_parallel_variant_mcsteps_and_fields():
+    # Two sweeps at one field, a field row and an antiferromagnetic pair:
+    # spins 0-1 with J=-1, field h0=+0.5 on spin 0; P=2, J_perp=0, T=1e-10.
+    # Start slice 0 = [-1,-1], slice 1 = [+1,-1].
+    # sweep 1, slice 0: spin0 e = -2(-1)(0.5f) + -2(-1)(-1 * -1) = 1 + 2 = 3 > 0 -> +1;
+    #                   spin1 e = -2(-1)(-1 * +1) = -2 -> draw, stay -1.
+    #          slice 1: spin0 e = -2(1)(0.5) + -2(1)(-1 * -1) = -1 - 2 = -3 -> draw, stay;
+    #                   spin1 e = -2(-1)(-1 * 1) = -2 -> draw, stay.
+    # sweep 2, slice 0: spin0 (+1 now) e = -1 - 2 = -3 -> draw, stay; spin1 -> draw.
+    #          slice 1: as in sweep 1 -> 2 draws.  No flips; 3 + 4 = 7 draws.
+    libc = _libc_rand()
+    nbs = np.array([[[0, 0.5], [1, -1.0]], [[0, -1.0], [1, 0.0]]])
+    confs = np.array([[-1.0, 1.0], [-1.0, -1.0]])
+    libc.srand(1)
+    O.qa_parallel(np.array([1e6]), 2, 2, 1e-10, 2, confs, nbs, 1)
+    assert libc.rand() == 1649760492      # the 8th rand() after srand(1)
+    assert confs.tolist() == [[1.0, 1.0], [-1.0, -1.0]]
+
+
 def test_trace_sa_per_spin_reset_and_ge():
     # Chain 0-1-2, J=+1, rows padded with [0,0]; svec [+1,+1,-1]; perm [1,0,2].
     # spin1: e=-2*(1)+(-2)*(-1)=0 -> ">=" accepts without a draw -> [+1,-1,-1]
