@@ -23,7 +23,9 @@ sys.path.insert(0, os.path.join(ROOT, "pathintegral-qmc_amd"))
 
 LDS_PEAK_GBS = 128 * 256 * 2.4  # 128 B/clk/CU x 256 CU x 2.4 GHz (nominal)
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_ATTEMPT = 8  # int8 spin traffic of one 2-D attempt: self + 4 + 2 Trotter + write
+# VALU issue peak: 1024 SIMDs x 16 lanes x 2.4 GHz = 39.3e12 32-bit lane operations/s
+# (a wave64 VALU instruction occupies its SIMD for 4 cycles)
+VALU_PEAK_TOPS = 1024 * 16 * 2.4e9 / 1e12
 
 
 COUNTERS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
@@ -31,6 +33,8 @@ COUNTERS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
 
 
 ISSUE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "colored_pmj1024_issue.md")
+
+WORK = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "colored_pmj1024_work.md")
 
 
 def _profile_figure(path, tag, R, N, P, S):
@@ -62,34 +66,114 @@ def measured_valu_issue(R, N, P, S):
     return _profile_figure(ISSUE, "valu_issue_fraction", R, N, P, S)
 
 
-def cpu_baseline(nbs, N, P, T, sched, reps):
+def algorithmic_valu_ops(R, N, P, S):
+    """(words per launch, minimal VALU lane-operations per word) of the +-1
+    sweep on this workload, from the committed work summary (the oracle's
+    counts of stochastic words and early-exit Philox blocks over sampled
+    replicas, priced at 23 lane-ops per word plus 48 per Philox block); else
+    (None, None)."""
+    return (_profile_figure(WORK, "words_per_launch", R, N, P, S),
+            _profile_figure(WORK, "min_valu_lane_ops_per_word", R, N, P, S))
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores():
+    """CPU threads this process may use: the OMP_NUM_THREADS share when set,
+    else the affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(nbs, N, P, T, sched, reps=5):
+    """CPU baselines on this host, all restated in C by the oracle:
+    B1  reference QuantumAnneal (piqmc/qmc.pyx:76-136), one thread, float64
+        F-ordered confs as the examples pass them: the 64-spin workload
+        (L=8, P=20, T=0.05, 100 field steps) in full, and one replica of the
+        1024-spin workload over all 500 field steps; median of `reps` runs
+        after a warm-up;
+    B2  QuantumAnneal_parallel (piqmc/qmc.pyx:247-357, OpenMP guided, libc
+        rand) at 1, 2, 4, ... host threads on every 10th field step of the
+        1024-spin workload (median of 3);
+    B3  every host thread running its own B1 replica (ctypes releases the
+        GIL), every 5th field step of the 1024-spin workload (median of 3)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import concurrent.futures as cf
     import numpy as np
     import oracle as O
+    from piqmc import instances
 
-    # reference QuantumAnneal restated in C, one thread, F-ordered float64 confs
-    tot, att = 0.0, 0
-    for r in range(reps):
-        rng = np.random.RandomState(1000 + r)
-        sv = np.array([2 * rng.randint(2) - 1 for _ in range(N)], np.float64)
-        confs = np.tile(sv, (P, 1)).T
-        perms = O.MT(2000 + r).perm_stream(N, len(sched))
+    def b1_run(nbs_, n, p, temp, sch, seed):
+        rng = np.random.RandomState(1000 + seed)
+        sv = np.array([2 * rng.randint(2) - 1 for _ in range(n)], np.float64)
+        confs = np.tile(sv, (p, 1)).T
+        perms = O.MT(2000 + seed).perm_stream(n, len(sch))
+        rs = O.Type3(3000 + seed)
         t0 = time.perf_counter()
-        O.qa_compat(sched, 1, P, T, N, confs, nbs, perms, O.Type3(3000 + r))
-        tot += time.perf_counter() - t0
-        att += N * P * len(sched)
-    single = att / tot
-    # QuantumAnneal_parallel restated (OpenMP guided, libc rand), all host threads
-    nthr = os.cpu_count() or 1
-    sub = sched[:: max(1, len(sched) // 50)]
-    rng = np.random.RandomState(7)
-    sv = np.array([2 * rng.randint(2) - 1 for _ in range(N)], np.float64)
-    confs = np.tile(sv, (P, 1)).T
-    t0 = time.perf_counter()
-    O.qa_parallel(sub, 1, P, T, N, confs, nbs, nthr)
-    omp = N * P * len(sub) / (time.perf_counter() - t0)
-    return single, omp, nthr, "%d replica(s) x %d field steps of the 1024-spin workload (%d attempts)" % (
-        reps, len(sched), att)
+        O.qa_compat(sch, 1, p, temp, n, confs, nbs_, perms, rs)
+        return time.perf_counter() - t0
+
+    def median_rate(attempts, times):
+        return attempts / float(np.median(times))
+
+    # B1 on the 64-spin workload, in full
+    n1, r1, c1, v1 = instances.lattice_pm_j(8, 2, np.random.RandomState(1))
+    nbs1 = instances.neighbors(n1, r1, c1, v1)
+    s1 = np.linspace(3.0, 1e-8, 100)
+    b1_run(nbs1, n1, 20, np.float32(0.05), s1, 0)
+    c1_rate = median_rate(n1 * 20 * 100, [b1_run(nbs1, n1, 20, np.float32(0.05), s1, k) for k in range(reps)])
+    # B1 on one replica of the 1024-spin workload, all 500 field steps
+    b1_run(nbs, N, P, T, sched[::25], 0)
+    c2_rate = median_rate(N * P * len(sched), [b1_run(nbs, N, P, T, sched, k) for k in range(reps)])
+    # B2 at 1, 2, 4, ... host threads
+    cores = host_cores()
+    sub = sched[::10]
+    omp = []
+    nthr = 1
+    while True:
+        times = []
+        for k in range(3):
+            rng = np.random.RandomState(7 + k)
+            sv = np.array([2 * rng.randint(2) - 1 for _ in range(N)], np.float64)
+            confs = np.tile(sv, (P, 1)).T
+            t0 = time.perf_counter()
+            O.qa_parallel(sub, 1, P, T, N, confs, nbs, nthr)
+            times.append(time.perf_counter() - t0)
+        omp.append({"threads": nthr, "attempts_per_s": median_rate(N * P * len(sub), times)})
+        if nthr >= cores:
+            break
+        nthr = min(2 * nthr, cores)
+    # B3: every host thread runs its own replica
+    sub3 = sched[::5]
+    times = []
+    with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+        for k in range(3):
+            t0 = time.perf_counter()
+            list(ex.map(lambda q: b1_run(nbs, N, P, T, sub3, 100 + q), range(cores)))
+            times.append(time.perf_counter() - t0)
+    b3 = median_rate(cores * N * P * len(sub3), times)
+    return {"value": c2_rate, "unit": "attempts/s", "cores": 1, "kind": "port",
+            "sample": "B1: reference QuantumAnneal restated in C (oracle), one thread, one replica of the "
+                      "1024-spin workload over all 500 field steps (%d attempts), median of %d after a "
+                      "warm-up" % (N * P * len(sched), reps),
+            "cpu_model": _cpu_model(), "host_threads": cores,
+            "b1_64spin_full": {"attempts_per_s": c1_rate, "attempts": n1 * 20 * 100,
+                               "workload": "2-D +-J L=8, P=20, T=0.05, 100 field steps, median of %d" % reps},
+            "b2_openmp": {"runs": omp, "workload": "QuantumAnneal_parallel restated (OpenMP guided, libc "
+                                                   "rand), every 10th field step of the 1024-spin workload"},
+            "b3_replica_parallel": {"attempts_per_s": b3, "threads": cores,
+                                    "workload": "one B1 replica per host thread, every 5th field step"}}
 
 
 def batch_leg(n_inst=512, reps=4, seed=0):
@@ -206,11 +290,13 @@ def main():
                     help="one anneal plus a 1 GiB device copy (for rocprofv3 --pmc runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-replay", action="store_true")
-    ap.add_argument("--replay-steps", type=int, default=25)
+    ap.add_argument("--replay-steps", type=int, default=500)
     ap.add_argument("--no-batch", action="store_true", help="skip the heterogeneous batch leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
-    ap.add_argument("--c5", action="store_true",
-                    help="also run the 3-D L=12 field workload (P=128, 1024 replicas, 2000 steps)")
+    ap.add_argument("--no-c5", action="store_true",
+                    help="skip the 3-D L=12 field workload (P=128, 1024 replicas, 2000 steps)")
+    ap.add_argument("--c5-learned", action="store_true",
+                    help="also run the learned proposal on the 3-D L=12 field workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the sweep shards replicas with no exchange, so "
@@ -265,7 +351,8 @@ def main():
     value = attempts * args.steps / dt
     kavg = sum(kms) / len(kms) / 1e3
     k_rate = attempts / kavg
-    achieved = BYTES_PER_ATTEMPT * attempts / kavg / 1e9
+    words, ops_per_word = algorithmic_valu_ops(R, N, P, S)
+    achieved = words * ops_per_word / kavg / 1e12 if words else None
     E = pb.slice_energies(confs)
     e_min = float(E.min(1).values.mean()) / N
 
@@ -285,18 +372,27 @@ def main():
         "config": {"workload": "2-D +-J glass L=32 N=1024 periodic, P=64, T=1/64, 4096 replicas, "
                                "Gamma 3.0->1e-8 over 500 steps x 1 sweep (parallel sweep)",
                    "replicas": R, "nspins": N, "slices": P, "field_steps": S, "sweeps_per_step": 1},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(R, N, P, S),
+        # The sweep is bound by VALU issue: `achieved` is the algorithm's own
+        # VALU work (minimal lane-operations per 32-attempt word times the
+        # words of one launch) over the kernel time, against the VALU issue
+        # peak; `valu_issue_frac` is what the SIMDs actually issued
+        # (rocprofv3 SQ_INSTS_VALU), so valu_issue_frac / frac is the
+        # instruction overhead over the minimal algorithm.
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS,
+                     "unit": "TOP/s (32-bit VALU lane operations)",
+                     "frac": achieved / VALU_PEAK_TOPS if achieved else None,
+                     "traffic": measured_traffic(R, N, P, S),
                      "kernel": "colored_q_kernel<uint32, degree 4, cached neighbours>", "tile": {"block": tile[0], "rpw": tile[1]},
                      "kernel_ms": kavg * 1e3, "kernel_attempts_per_s": k_rate,
-                     "bytes_per_attempt": BYTES_PER_ATTEMPT,
-                     "lds_frac": achieved / LDS_PEAK_GBS,
-                     "valu_issue_frac": measured_valu_issue(R, N, P, S)},
+                     "words_per_launch": words, "min_valu_lane_ops_per_word": ops_per_word,
+                     "valu_issue_frac": measured_valu_issue(R, N, P, S),
+                     "hbm_frac": (measured_traffic(R, N, P, S) or 0) / kavg / 1e9 / HBM_PEAK_GBS},
         "mean_min_slice_energy_per_spin": e_min,
     }
 
     if not args.no_replay:
-        # bit-exact replay on the same instance, all replicas, fewer field steps
+        # bit-exact replay on the same instance and replicas, the whole
+        # 500-step schedule (a warm-up over its first 5 field steps)
         Sr = args.replay_steps
         sr = np.linspace(3.0, 1e-8, Sr)
         perms = torch.from_numpy(device.perm_streams(np.arange(R, dtype=np.uint32) + 1, N, Sr)
@@ -304,7 +400,7 @@ def main():
         rs0 = torch.from_numpy(device.rand_states(np.arange(R) + 7).view(np.int32)).to(dev)
         rs = rs0.clone()
         c2 = init.clone()
-        pb.qa_replay(c2, sr, 1, T, perms, rs)          # warm-up
+        pb.qa_replay(c2, sr[:5], 1, T, perms, rs)          # warm-up
         torch.cuda.synchronize()
         c2.copy_(init)
         rs.copy_(rs0)
@@ -321,9 +417,9 @@ def main():
 
     if not args.no_batch:
         out["batch"] = batch_leg()
-    if args.c5:
+    if not args.no_c5:
         out["c5"] = c5_leg()
-        if not args.no_learned:
+        if args.c5_learned:
             from piqmc import learned
             out["c5_learned"] = learned.run(L=12, dim=3, field_sigma=0.1, P=128, R=1024, hidden=1024,
                                             train_steps=400, epochs=2, eval_steps=(25, 50, 100),
@@ -337,14 +433,11 @@ def main():
                                              "curves", "time_ratio")}
 
     if not args.no_cpu:
-        single, omp, nthr, sample = cpu_baseline(nbs, N, P, np.float32(T), sched, reps=2)
-        out["cpu_baseline"] = {"value": single, "unit": "attempts/s", "cores": 1, "kind": "port",
-                               "sample": "reference QuantumAnneal restated in C (oracle), " + sample}
-        out["cpu_openmp"] = {"value": omp, "unit": "attempts/s", "cores": nthr, "kind": "port",
-                             "sample": "QuantumAnneal_parallel restated (OpenMP guided, libc rand)"}
-        out["speedup_vs_cpu"] = value / single
+        cb = cpu_baseline(nbs, N, P, np.float32(T), sched)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = value / cb["value"]
         if "replay" in out:
-            out["replay"]["speedup_vs_cpu"] = out["replay"]["attempts_per_s"] / single
+            out["replay"]["speedup_vs_cpu"] = out["replay"]["attempts_per_s"] / cb["value"]
     print(json.dumps(out))
 
 
