@@ -169,6 +169,19 @@ int piqmc_qa_colored(const piqmc_problem *pb, int R, int P, int S, int M, float 
                      const double *sched_host, int8_t *confs_dev, uint64_t seed,
                      uint32_t replica_base, uint32_t sweep_base, double *etrace_dev,
                      int block, int rpw, void *stream);
+/* piqmc_qa_colored that allocates no device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as piqmc_qa_colored; the per-call buffers
+ * (J_perp and temperature tables, the +-1 threshold tables, the packed state
+ * [R][2][N] words) come from a fixed 64 MiB device region per GPU, held until
+ * the process ends.  A call needing more (e.g. more than ~8000 replicas of
+ * 1024 spins at P <= 64) fails with PIQMC_EINVAL.  All calls on one GPU share
+ * that region: calls that overlap on one GPU (other streams or threads) race. */
+int piqmc_qa_colored_noalloc(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
+                             const double *sched_host, int8_t *confs_dev, uint64_t seed,
+                             uint32_t replica_base, uint32_t sweep_base, double *etrace_dev,
+                             int block, int rpw, void *stream);
 /* Classical annealing with the parallel sweep (the preanneal of
  * examples/spinglass32.py:121, sa.Anneal, on the GPU): C independent chains
  * spins_dev int8 [C][N] (C even) in place, temperature sched_host[s] in field

@@ -25,6 +25,8 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include "piqmc_internal.h"
@@ -32,6 +34,17 @@
 namespace {
 
 constexpr int kClassStride = 8 * 8 * 3;  // (z, u_s, u_t) thresholds per field step
+
+// Device scratch of the allocation-free parallel sweep (piqmc_qa_colored_noalloc):
+// one copy per GPU, held for the life of the process.  Calls on one GPU share
+// it, so calls that overlap on one GPU race.  Every buffer carved from it is
+// fully written before it is read (tables by copies, the packed state by the
+// pack kernel), so nothing in it needs clearing between calls.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr int kScratchMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+unsigned char *g_scratch_ptr[kScratchMaxDevices] = {};
+std::mutex g_scratch_mu;
 // Stochastic-word queues of the +-1 sweep, per wavefront, never holding more
 // than 63 + 64 records between drains: with several classes one queue of
 // 16-byte records (index | replica << 16, three class masks); with a single
@@ -1697,12 +1710,67 @@ bool use_dense(const piqmc_problem *pb)
     return pb->sweep_path == 2 || (pb->sweep_path == 0 && pb->N < 8 * pb->ncolors);
 }
 
+// The device buffers of one parallel-sweep call.  Either they are carved,
+// 256-byte aligned, out of a region the caller owns (base != null: nothing is
+// allocated), or each is allocated on the call's stream and freed there when
+// the call returns, on its error paths included.
+struct CallBuffers {
+    unsigned char *base = nullptr;
+    size_t cap = 0, used = 0;
+    hipStream_t st = nullptr;
+    void *owned[8] = {};
+    int nowned = 0;
+
+    static size_t round_up(size_t b) { return (b + 255) & ~(size_t)255; }
+    int take(void **p, size_t bytes)
+    {
+        *p = nullptr;
+        if (base) {
+            if (used + round_up(bytes) > cap) return piqmc::set_error(PIQMC_EINVAL, "parallel sweep: scratch region too small");
+            *p = base + used;
+            used += round_up(bytes);
+            return PIQMC_OK;
+        }
+        if (nowned == 8) return piqmc::set_error(PIQMC_EINVAL, "parallel sweep: too many buffers");
+        PIQMC_HIP(hipMallocAsync(p, bytes, st));
+        owned[nowned++] = *p;
+        return PIQMC_OK;
+    }
+    ~CallBuffers()
+    {
+        for (int k = nowned - 1; k >= 0; --k) (void)hipFreeAsync(owned[k], st);
+    }
+};
+
+// Device bytes a parallel-sweep call takes from a CallBuffers region (each
+// buffer rounded up to 256 bytes): J_perp and temperatures [2S] floats, the
+// +-1 path's per-step tables, the color offsets and the packed state
+// [R][2][N] words.
+size_t colored_scratch_bytes(const piqmc_problem *pb, int R, int P, int S)
+{
+    const size_t wb = P / 2 <= 32 ? 4 : 8;
+    size_t b = CallBuffers::round_up(sizeof(float) * (S ? 2 * S : 1));
+    if (!use_dense(pb)) {
+        if (pb->pmj && S)
+            b += CallBuffers::round_up(sizeof(uint32_t) * (size_t)S * kClassStride) +
+                 CallBuffers::round_up(sizeof(uint32_t) * (size_t)S * 8);
+        b += CallBuffers::round_up(sizeof(int) * (pb->ncolors + 1));
+    }
+    return b + CallBuffers::round_up(wb * 2 * (size_t)pb->N * R);
+}
+
 template <typename WT>
 int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
                 const std::vector<float> &jp, const std::vector<float> &temps, int8_t *confs, uint64_t seed, uint32_t rbase,
-                uint32_t sbase, double *etrace, int block, int rpw, hipStream_t st)
+                uint32_t sbase, double *etrace, int block, int rpw, hipStream_t st,
+                unsigned char *scratch = nullptr, size_t scratch_cap = 0)
 {
     const int N = pb->N;
+    CallBuffers buf;
+    buf.base = scratch;
+    buf.cap = scratch_cap;
+    buf.st = st;
+    int rc;
     // dense path when color classes average fewer than 8 spins (or forced)
     if (use_dense(pb)) {
         // one replica per wavefront; small replica counts get one wavefront per
@@ -1721,10 +1789,10 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         if (!wg && lds > 160 * 1024) return piqmc::set_error(PIQMC_EINVAL, "dense sweep: state exceeds LDS");
         float *d_jp = nullptr;  // [S] J_perp, then [S] temperatures
         WT *d_packed = nullptr;
-        PIQMC_HIP(hipMallocAsync((void **)&d_jp, sizeof(float) * (S ? 2 * S : 1), st));
+        if ((rc = buf.take((void **)&d_jp, sizeof(float) * (S ? 2 * S : 1)))) return rc;
         if (S) PIQMC_HIP(hipMemcpyAsync(d_jp, jp.data(), sizeof(float) * S, hipMemcpyHostToDevice, st));
         if (S) PIQMC_HIP(hipMemcpyAsync(d_jp + S, temps.data(), sizeof(float) * S, hipMemcpyHostToDevice, st));
-        PIQMC_HIP(hipMallocAsync((void **)&d_packed, sizeof(WT) * 2 * (size_t)N * R, st));
+        if ((rc = buf.take((void **)&d_packed, sizeof(WT) * 2 * (size_t)N * R))) return rc;
         launch_pack<WT>(confs, d_packed, R, P, N, st);
         ColArgs<WT> a = {};
         a.N = N; a.Z = pb->Z; a.P = P; a.W = P / 2; a.S = S; a.M = M; a.R = R;
@@ -1755,8 +1823,6 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         piqmc::timing_end(st);
         PIQMC_HIP(hipGetLastError());
         launch_unpack<WT>(d_packed, confs, R, P, N, st);
-        PIQMC_HIP(hipFreeAsync(d_packed, st));
-        PIQMC_HIP(hipFreeAsync(d_jp, st));
         PIQMC_HIP(hipGetLastError());
         return PIQMC_OK;
     }
@@ -1780,18 +1846,22 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
     uint32_t *d_thr = nullptr, *d_zt = nullptr;
     int *d_coff = nullptr;
     WT *d_packed = nullptr;
-    PIQMC_HIP(hipMallocAsync((void **)&d_jp, sizeof(float) * (S ? 2 * S : 1), st));
+    if ((rc = buf.take((void **)&d_jp, sizeof(float) * (S ? 2 * S : 1)))) return rc;
     if (S) PIQMC_HIP(hipMemcpyAsync(d_jp, jp.data(), sizeof(float) * S, hipMemcpyHostToDevice, st));
     if (S) PIQMC_HIP(hipMemcpyAsync(d_jp + S, temps.data(), sizeof(float) * S, hipMemcpyHostToDevice, st));
-    if (pmj && S) {
-        PIQMC_HIP(hipMallocAsync((void **)&d_thr, sizeof(uint32_t) * thr.size(), st));
-        PIQMC_HIP(hipMallocAsync((void **)&d_zt, sizeof(uint32_t) * ztab.size(), st));
-        PIQMC_HIP(hipMemcpyAsync(d_thr, thr.data(), sizeof(uint32_t) * thr.size(), hipMemcpyHostToDevice, st));
-        PIQMC_HIP(hipMemcpyAsync(d_zt, ztab.data(), sizeof(uint32_t) * ztab.size(), hipMemcpyHostToDevice, st));
+    if (pb->pmj && S) {
+        // taken whenever the problem is +-1 (as colored_scratch_bytes counts),
+        // filled only when the tables apply
+        if ((rc = buf.take((void **)&d_thr, sizeof(uint32_t) * (size_t)S * kClassStride))) return rc;
+        if ((rc = buf.take((void **)&d_zt, sizeof(uint32_t) * (size_t)S * 8))) return rc;
+        if (pmj) {
+            PIQMC_HIP(hipMemcpyAsync(d_thr, thr.data(), sizeof(uint32_t) * thr.size(), hipMemcpyHostToDevice, st));
+            PIQMC_HIP(hipMemcpyAsync(d_zt, ztab.data(), sizeof(uint32_t) * ztab.size(), hipMemcpyHostToDevice, st));
+        }
     }
-    PIQMC_HIP(hipMallocAsync((void **)&d_coff, sizeof(int) * (pb->ncolors + 1), st));
+    if ((rc = buf.take((void **)&d_coff, sizeof(int) * (pb->ncolors + 1)))) return rc;
     PIQMC_HIP(hipMemcpyAsync(d_coff, pb->color_off.data(), sizeof(int) * (pb->ncolors + 1), hipMemcpyHostToDevice, st));
-    PIQMC_HIP(hipMallocAsync((void **)&d_packed, sizeof(WT) * 2 * (size_t)N * R, st));
+    if ((rc = buf.take((void **)&d_packed, sizeof(WT) * 2 * (size_t)N * R))) return rc;
     launch_pack<WT>(confs, d_packed, R, P, N, st);
 
     ColArgs<WT> a;
@@ -1801,7 +1871,7 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
     a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
     a.replica_base = rbase; a.sweep_base = sbase;
     a.jperp = d_jp; a.rows = pb->d_rows; a.order = pb->d_order; a.color_off = d_coff;
-    a.cnb = pb->d_cnb; a.cinfo = pb->d_cinfo; a.thr = d_thr; a.ztab = d_zt; a.hq = pb->d_hq; a.prevj = pb->d_prevj;
+    a.cnb = pb->d_cnb; a.cinfo = pb->d_cinfo; a.thr = pmj ? d_thr : nullptr; a.ztab = pmj ? d_zt : nullptr; a.hq = pb->d_hq; a.prevj = pb->d_prevj;
     a.packed = d_packed; a.etrace = etrace;
     const int grid = (R + rpw - 1) / rpw;
     piqmc::timing_begin(st);
@@ -1840,20 +1910,60 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
     piqmc::timing_end(st);
     PIQMC_HIP(hipGetLastError());
     launch_unpack<WT>(d_packed, confs, R, P, N, st);
-    PIQMC_HIP(hipFreeAsync(d_packed, st));
-    PIQMC_HIP(hipFreeAsync(d_coff, st));
-    if (d_thr) PIQMC_HIP(hipFreeAsync(d_thr, st));
-    if (d_zt) PIQMC_HIP(hipFreeAsync(d_zt, st));
-    PIQMC_HIP(hipFreeAsync(d_jp, st));
     PIQMC_HIP(hipGetLastError());
     return PIQMC_OK;
 }
 
 }  // namespace
 
+namespace {
+int qa_colored_impl(const piqmc_problem *pb, int R, int P, int S, int M, float temp, const double *sched,
+                    int8_t *confs, uint64_t seed, uint32_t rbase, uint32_t sbase, double *etrace, int block,
+                    int rpw, hipStream_t st, bool use_scratch);
+}
+
 extern "C" int piqmc_qa_colored(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
                                 const double *sched, int8_t *confs, uint64_t seed, uint32_t rbase,
                                 uint32_t sbase, double *etrace, int block, int rpw, void *stream)
+{
+    return qa_colored_impl(pb, R, P, S, M, temp, sched, confs, seed, rbase, sbase, etrace, block, rpw,
+                           (hipStream_t)stream, false);
+}
+
+extern "C" int piqmc_qa_colored_noalloc(const piqmc_problem *pb, int R, int P, int S, int M, float temp,
+                                        const double *sched, int8_t *confs, uint64_t seed, uint32_t rbase,
+                                        uint32_t sbase, double *etrace, int block, int rpw, void *stream)
+{
+    return qa_colored_impl(pb, R, P, S, M, temp, sched, confs, seed, rbase, sbase, etrace, block, rpw,
+                           (hipStream_t)stream, true);
+}
+
+namespace {
+
+// Address of this device's copy of `scratch`, looked up once per device with
+// that device current.  The host-side name of the array is not a device
+// address; only these pointers are passed to kernels and copies.
+int scratch_region(unsigned char **out)
+{
+    int dev = -1;
+    PIQMC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchMaxDevices)
+        return piqmc::set_error(PIQMC_EINVAL, "qa_colored_noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return piqmc::hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return piqmc::set_error(PIQMC_EHIP, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return PIQMC_OK;
+}
+
+int qa_colored_impl(const piqmc_problem *pb, int R, int P, int S, int M, float temp, const double *sched,
+                    int8_t *confs, uint64_t seed, uint32_t rbase, uint32_t sbase, double *etrace, int block,
+                    int rpw, hipStream_t st, bool use_scratch)
 {
     if (!pb || R < 1 || S < 0 || M < 0 || (S > 0 && !sched) || !confs)
         return piqmc::set_error(PIQMC_EINVAL, "qa_colored: bad arguments");
@@ -1865,12 +1975,28 @@ extern "C" int piqmc_qa_colored(const piqmc_problem *pb, int R, int P, int S, in
         jp[s] = host_jperp(sched[s], P, temp);
     }
     if ((long)S * M > 0 && temp == 0.0f) return piqmc::set_error(PIQMC_EZERODIV, "float division");
-    hipStream_t st = (hipStream_t)stream;
+    unsigned char *region = nullptr;
+    size_t cap = 0;
+    if (use_scratch) {
+        const size_t need = colored_scratch_bytes(pb, R, P, S);
+        if (need > kScratchBytes)
+            return piqmc::set_error(PIQMC_EINVAL, "qa_colored_noalloc: this call needs " + std::to_string(need) +
+                                                      " bytes of device scratch; the maximum is " +
+                                                      std::to_string(kScratchBytes) +
+                                                      " (use piqmc_qa_colored, or fewer replicas per call)");
+        int rc = scratch_region(&region);
+        if (rc) return rc;
+        cap = kScratchBytes;
+    }
     const std::vector<float> temps(jp.size(), temp);
     if (P / 2 <= 32)
-        return run_colored<uint32_t>(pb, R, P, S, M, jp, temps, confs, seed, rbase, sbase, etrace, block, rpw, st);
-    return run_colored<uint64_t>(pb, R, P, S, M, jp, temps, confs, seed, rbase, sbase, etrace, block, rpw, st);
+        return run_colored<uint32_t>(pb, R, P, S, M, jp, temps, confs, seed, rbase, sbase, etrace, block, rpw, st,
+                                     region, cap);
+    return run_colored<uint64_t>(pb, R, P, S, M, jp, temps, confs, seed, rbase, sbase, etrace, block, rpw, st,
+                                 region, cap);
 }
+
+}  // namespace
 
 // Classical annealing of C independent chains (C even) with the parallel
 // sweep: chains are laid out as the slices of C/P' replicas (P' the largest

@@ -48,6 +48,8 @@ _SIGS = {
     "piqmc_sa_colored": ([_vp, _i, _i, _i, _vp, _vp, _u64, _u32, _u32, _i, _i, _vp], _i),
     "piqmc_qa_colored": ([_vp, _i, _i, _i, _i, _f, _vp, _vp, _u64, _u32, _u32, _vp, _i, _i, _vp],
                          _i),
+    "piqmc_qa_colored_noalloc": ([_vp, _i, _i, _i, _i, _f, _vp, _vp, _u64, _u32, _u32, _vp, _i, _i,
+                                  _vp], _i),
     "piqmc_slice_energies": ([_vp, _i, _vp, _vp, _vp], _i),
     "piqmc_classical_energy": ([_i, _i64, _vp, _vp, _vp, _i, _vp, _vp], _i),
     "piqmc_generate_neighbors": ([_i, _i, _i64, _vp, _vp, _vp, _i, _vp], _i),
