@@ -182,28 +182,17 @@ def batch_leg(n_inst=512, reps=4, seed=0):
     uniform in [100,1000], T = 1/P: one persistent launch."""
     import numpy as np
     import torch
-    from piqmc import device, instances
-    rng = np.random.RandomState(seed)
-    kinds = [("pm", 8), ("pm", 16), ("pm", 32), ("pm", 64), ("sk", 64), ("sk", 128), ("sk", 256)]
+    from piqmc import device, workloads
     probs, b, attempts, mix = {}, device.Batch(), 0, {}
-    for t in range(n_inst):
-        kind, size = kinds[rng.randint(len(kinds))]
-        P = int(rng.choice([16, 32, 64, 128]))
-        S = int(rng.randint(100, 1001))
-        key = (kind, size, t % 4)          # a few distinct instances per size
-        if key not in probs:
-            iseed = (1000 if kind == "sk" else 0) + 10 * size + t % 4
-            if kind == "sk":
-                n, r, c, v = instances.sk(size, np.random.RandomState(iseed))
-            else:
-                n, r, c, v = instances.lattice_pm_j(size, 2, np.random.RandomState(iseed))
-            probs[key] = device.Problem(instances.neighbors(n, r, c, v))
-        pb = probs[key]
-        g = torch.Generator(device="cpu").manual_seed(t)
-        confs = (torch.randint(0, 2, (reps, P, pb.nspins), generator=g, dtype=torch.int8) * 2 - 1).cuda()
-        b.add(pb, confs, np.linspace(3.0, 1e-8, S), 1, 1.0 / P, 1000 + t, replica_base=t * reps)
-        attempts += reps * P * pb.nspins * S
-        mix[kind] = mix.get(kind, 0) + 1
+    for job in workloads.batch_mix(n_inst, reps, seed):
+        if job["key"] not in probs:
+            probs[job["key"]] = device.Problem(workloads.batch_instance(job)[1])
+        pb = probs[job["key"]]
+        confs = workloads.batch_init(job, pb.nspins).cuda()
+        b.add(pb, confs, np.linspace(3.0, 1e-8, job["S"]), 1, job["T"], job["seed"],
+              replica_base=job["replica_base"])
+        attempts += reps * job["P"] * pb.nspins * job["S"]
+        mix[job["kind"]] = mix.get(job["kind"], 0) + 1
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     b.run()
@@ -400,7 +389,7 @@ def main():
         rs0 = torch.from_numpy(device.rand_states(np.arange(R) + 7).view(np.int32)).to(dev)
         rs = rs0.clone()
         c2 = init.clone()
-        pb.qa_replay(c2, sr[:5], 1, T, perms, rs)          # warm-up
+        pb.qa_replay(c2, sr[:5], 1, T, perms[:5], rs)      # warm-up
         torch.cuda.synchronize()
         c2.copy_(init)
         rs.copy_(rs0)

@@ -6,6 +6,8 @@
 //    E = -s.(J_off s) - sum(diag(J) s), over a COO coupling list.
 // One workgroup per configuration; per-thread partial sums are reduced with
 // wavefront shuffles and one LDS slot per wave.
+#include <mutex>
+
 #include "piqmc_internal.h"
 
 namespace {
@@ -75,6 +77,20 @@ extern "C" int piqmc_slice_energies(const piqmc_problem *pb, int B, const int8_t
     return PIQMC_OK;
 }
 
+namespace {
+// Device workspace of piqmc_classical_energy, kept between calls (one per
+// device, grown when a call needs more) so that repeated energy evaluations
+// do not allocate; calls are serialised by its mutex.
+struct EnergyWorkspace {
+    std::mutex mu;
+    void *buf = nullptr;
+    size_t cap = 0;
+    hipStream_t st = nullptr;
+};
+constexpr int kEnergyMaxDevices = 64;
+EnergyWorkspace g_energy_ws[kEnergyMaxDevices];
+}  // namespace
+
 extern "C" int piqmc_classical_energy(int N, int64_t nnz, const int64_t *rows, const int64_t *cols,
                                       const double *vals, int B, const double *spins, double *E)
 {
@@ -84,28 +100,36 @@ extern "C" int piqmc_classical_energy(int N, int64_t nnz, const int64_t *rows, c
         if (rows[t] < 0 || rows[t] >= N || cols[t] < 0 || cols[t] >= N)
             return piqmc::set_error(PIQMC_EINVAL, "coupling index out of range");
     if (B == 0) return PIQMC_OK;
-    int64_t *d_r = nullptr, *d_c = nullptr;
-    double *d_v = nullptr, *d_s = nullptr, *d_e = nullptr;
+    int dev = -1;
+    PIQMC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kEnergyMaxDevices) return piqmc::set_error(PIQMC_EINVAL, "classical_energy: device ordinal");
+    EnergyWorkspace &ws = g_energy_ws[dev];
+    std::lock_guard<std::mutex> lock(ws.mu);
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t nz = nnz ? (size_t)nnz : 1;
-    hipError_t e = hipMalloc(&d_r, 8 * nz);
-    if (e == hipSuccess) e = hipMalloc(&d_c, 8 * nz);
-    if (e == hipSuccess) e = hipMalloc(&d_v, 8 * nz);
-    if (e == hipSuccess) e = hipMalloc(&d_s, 8 * (size_t)B * (N ? N : 1));
-    if (e == hipSuccess) e = hipMalloc(&d_e, 8 * (size_t)B);
-    if (e == hipSuccess && nnz) e = hipMemcpy(d_r, rows, 8 * nz, hipMemcpyHostToDevice);
-    if (e == hipSuccess && nnz) e = hipMemcpy(d_c, cols, 8 * nz, hipMemcpyHostToDevice);
-    if (e == hipSuccess && nnz) e = hipMemcpy(d_v, vals, 8 * nz, hipMemcpyHostToDevice);
-    if (e == hipSuccess && N) e = hipMemcpy(d_s, spins, 8 * (size_t)B * N, hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
-        coo_energy_kernel<<<B, 256>>>(d_r, d_c, d_v, nnz, N, d_s, d_e);
-        e = hipGetLastError();
+    const size_t o_c = up(8 * nz), o_v = o_c + up(8 * nz), o_s = o_v + up(8 * nz);
+    const size_t o_e = o_s + up(8 * (size_t)B * (N ? N : 1)), need = o_e + up(8 * (size_t)B);
+    if (!ws.st) PIQMC_HIP(hipStreamCreateWithFlags(&ws.st, hipStreamNonBlocking));
+    if (need > ws.cap) {
+        if (ws.buf) PIQMC_HIP(hipFree(ws.buf));
+        ws.buf = nullptr;
+        ws.cap = 0;
+        PIQMC_HIP(hipMalloc(&ws.buf, need));
+        ws.cap = need;
     }
-    if (e == hipSuccess) e = hipMemcpy(E, d_e, 8 * (size_t)B, hipMemcpyDeviceToHost);
-    (void)hipFree(d_r);
-    (void)hipFree(d_c);
-    (void)hipFree(d_v);
-    (void)hipFree(d_s);
-    (void)hipFree(d_e);
-    if (e != hipSuccess) return piqmc::hip_fail(e, "classical_energy");
+    char *base = static_cast<char *>(ws.buf);
+    int64_t *d_r = reinterpret_cast<int64_t *>(base), *d_c = reinterpret_cast<int64_t *>(base + o_c);
+    double *d_v = reinterpret_cast<double *>(base + o_v), *d_s = reinterpret_cast<double *>(base + o_s);
+    double *d_e = reinterpret_cast<double *>(base + o_e);
+    if (nnz) {
+        PIQMC_HIP(hipMemcpyAsync(d_r, rows, 8 * nz, hipMemcpyHostToDevice, ws.st));
+        PIQMC_HIP(hipMemcpyAsync(d_c, cols, 8 * nz, hipMemcpyHostToDevice, ws.st));
+        PIQMC_HIP(hipMemcpyAsync(d_v, vals, 8 * nz, hipMemcpyHostToDevice, ws.st));
+    }
+    if (N) PIQMC_HIP(hipMemcpyAsync(d_s, spins, 8 * (size_t)B * N, hipMemcpyHostToDevice, ws.st));
+    coo_energy_kernel<<<B, 256, 0, ws.st>>>(d_r, d_c, d_v, nnz, N, d_s, d_e);
+    PIQMC_HIP(hipGetLastError());
+    PIQMC_HIP(hipMemcpyAsync(E, d_e, 8 * (size_t)B, hipMemcpyDeviceToHost, ws.st));
+    PIQMC_HIP(hipStreamSynchronize(ws.st));
     return PIQMC_OK;
 }

@@ -751,7 +751,11 @@ extern "C" int piqmc_made_mh(piqmc_made *m, const piqmc_problem *pb, int R, int 
     PIQMC_HIP(hipMallocAsync((void **)&lq_old, 8 * (size_t)R, st));
     gather_slices<<<grid1((size_t)R * m->N), 256, 0, st>>>(confs, R, P, m->N, slice, old);
     int rc = piqmc_made_logprob(m, R, old, lq_old, stream);
-    if (rc) return rc;
+    if (rc) {
+        (void)hipFreeAsync(old, st);
+        (void)hipFreeAsync(lq_old, st);
+        return rc;
+    }
     mh_kernel<<<R, 256, 0, st>>>(confs, R, P, m->N, pb->Z, pb->d_rows, newsl, lq_new, lq_old, slice, jperp, temp,
                                  (uint32_t)seed, (uint32_t)(seed >> 32), round, accepted);
     PIQMC_HIP(hipFreeAsync(old, st));

@@ -107,14 +107,20 @@ def _quantum_anneal(sched, mcsteps, slices, temp, nspins, confs, cpl, rng, dense
     return None
 
 
+_parallel_seeds = np.random.default_rng()
+
+
 def QuantumAnneal_parallel(sched, mcsteps, slices, temp, nspins, confs, nbs, nthreads,
                            seed=None):
     """QuantumAnneal_parallel(sched, mcsteps, slices, temp, nspins, confs, nbs, nthreads)
 
     Parallel sweep on the GPU in place on ``confs`` [nspins, slices]
     (piqmc/qmc.pyx:247-357 signature; ``nthreads`` is accepted and ignored).
-    ``seed`` keys the Philox stream (drawn from numpy's global generator when
-    None).  Requires an even number of slices.
+    ``seed`` keys the Philox stream; when None it comes from a generator
+    private to this module, so numpy's global RandomState is left untouched.
+    The sweep is the race-free graph-colored one: the reference's OpenMP
+    variant is nondeterministic (racy reads of neighbouring spins), so no call
+    can reproduce it.  Requires an even number of slices.
     """
     from . import device
     import torch
@@ -128,7 +134,7 @@ def QuantumAnneal_parallel(sched, mcsteps, slices, temp, nspins, confs, nbs, nth
     if not np.all(np.abs(confs) == 1.0):
         raise ValueError("spin configurations must hold +1/-1 values")
     if seed is None:
-        seed = int(np.random.randint(0, 2**31 - 1))
+        seed = int(_parallel_seeds.integers(0, 2**63 - 1))
     pb = device.Problem(nbs)
     c8 = torch.from_numpy(np.ascontiguousarray(confs.T).astype(np.int8))[None].cuda()
     pb.qa_colored(c8, sched, mcsteps, c_float(temp), seed)

@@ -241,6 +241,26 @@ def test_instance_ground_state_energy():
     assert abs(E / 1024 - (-1.55460615142578)) < 1e-7
 
 
+def test_santoro_80x80_ground_state_energy():
+    # examples/ising_instances/santoro_80x80_answer.txt:24,36-256 -- exact GS of
+    # the 80x80 Gaussian instance, read 1-based (the indices list the +1
+    # spins; read 0-based, as examples/santoro80.py:263 does, the energy would
+    # be +0.0066 per spin)
+    a = np.loadtxt(os.path.join(GOLDEN, "santoro_80x80.txt"))
+    ups = np.loadtxt(os.path.join(GOLDEN, "santoro_80x80_gs_up.txt"), dtype=int) - 1
+    assert len(ups) == 3184                              # "spins up: 3184" (answer.txt:34)
+    gs = -np.ones(6400)
+    gs[ups] = 1
+    r, c = a[:, 0].astype(np.int64) - 1, a[:, 1].astype(np.int64) - 1
+    E = O.energy_coo(gs, r, c, a[:, 2])
+    assert abs(E / 6400 - (-1.58051667679)) < 1e-9
+    # a ground state: no single spin flip lowers the energy
+    h = np.zeros(6400)
+    np.add.at(h, r, a[:, 2] * gs[c])
+    np.add.at(h, c, a[:, 2] * gs[r])
+    assert np.all(gs * h >= 0)
+
+
 # ------------------------------------------------------------ fixed semantics: stationarity
 def _exact_mean_energy(nbs, P, T, jp):
     """<(1/P) sum_k E_k> under exp(-A/T), A = sum_k E_k - Jp sum_k,i s^k s^{k+1},
