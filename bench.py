@@ -66,6 +66,15 @@ def measured_valu_issue(R, N, P, S):
     return _profile_figure(ISSUE, "valu_issue_fraction", R, N, P, S)
 
 
+def hit_energy(name):
+    """Best-known energy of a +-J workload instance (tests/golden/pmj_hit_energies.json)."""
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "pmj_hit_energies.json")) as f:
+            return json.load(f)[name]["energy"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def algorithmic_valu_ops(R, N, P, S):
     """(words per launch, minimal VALU lane-operations per word) of the +-1
     sweep on this workload, from the committed work summary (the oracle's
@@ -344,6 +353,7 @@ def main():
     achieved = words * ops_per_word / kavg / 1e12 if words else None
     E = pb.slice_energies(confs)
     e_min = float(E.min(1).values.mean()) / N
+    e32 = hit_energy("L32_seed2024")
 
     out = {
         "metric": "spin-flip attempts/s (1 GPU)",
@@ -377,6 +387,8 @@ def main():
                      "valu_issue_frac": measured_valu_issue(R, N, P, S),
                      "hbm_frac": (measured_traffic(R, N, P, S) or 0) / kavg / 1e9 / HBM_PEAK_GBS},
         "mean_min_slice_energy_per_spin": e_min,
+        "residual_per_spin": (e_min - e32 / N) if e32 is not None else None,
+        "best_known_energy_per_spin": (e32 / N) if e32 is not None else None,
     }
 
     if not args.no_replay:
@@ -416,10 +428,13 @@ def main():
     if not args.no_learned:
         out["mfma_gemm"] = gemm_leg()
         from piqmc import learned
-        lr = learned.run(eval_steps=(25, 50, 100, 200))
-        out["learned"] = {k: lr[k] for k in ("workload", "train_slices", "train_nll_per_epoch",
-                                             "train_seconds", "train_tflops", "e_best_per_spin",
-                                             "curves", "time_ratio")}
+        keys = ("workload", "move", "beta", "train_slices", "train_nll_per_epoch", "train_seconds",
+                "train_data_seconds", "train_tflops", "e_ref_per_spin", "e_ref", "curves", "time_ratio")
+        e16 = hit_energy("L16_seed1")
+        lr = learned.run(eval_steps=(25, 50, 100, 200), e_ref=e16, move="slice")
+        out["learned"] = {k: lr[k] for k in keys}
+        lw = learned.run(eval_steps=(25, 50, 100), e_ref=e16, move="worldline", beta=2.0, rounds=(1, 2, 4, 8))
+        out["learned_worldline"] = {k: lw[k] for k in keys}
 
     if not args.no_cpu:
         cb = cpu_baseline(nbs, N, P, np.float32(T), sched)

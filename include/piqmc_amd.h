@@ -81,6 +81,18 @@ int piqmc_problem_get_coloring(const piqmc_problem *pb, int32_t *color_host);
  * than 8 spins), 1 one parity word per thread, 2 one replica per wavefront
  * with a lane per slice.  Every choice gives identical results. */
 int piqmc_problem_set_sweep_path(piqmc_problem *pb, int path);
+/* Layout of the lane-per-slice (dense) sweep, the tile the autotuner picks
+ * for dense problems: 0 automatic; 1 one replica per wavefront, a lane per
+ * slice; 2 each slice's field split over 64/(P/2) lanes; 3 one replica per
+ * workgroup of four wavefronts (the field split over them); 4 / 5 that
+ * workgroup pipelined, one wavefront deciding a spin while three / seven sum
+ * the next one's field; 6 one replica per wavefront with every spin's local
+ * field in every slice of the swept parity kept in LDS and updated on flips
+ * (rows that list each neighbour once with the same coupling both ways).
+ * 2-6 need rows that sum exactly in any order (piqmc_problem_exact_rows); a
+ * sweep whose variant does not apply fails with PIQMC_EINVAL.  Every variant
+ * gives identical results.  No reference counterpart. */
+int piqmc_problem_set_dense_variant(piqmc_problem *pb, int variant);
 
 /* ------------------------------------------------------------------ */
 /* Random streams of the reference, for replay.                       */
@@ -252,6 +264,12 @@ int piqmc_made_logprob(piqmc_made *m, int B, const int8_t *s_dev, double *logq_d
  * (beta 0.9/0.999, eps 1e-8) when lr > 0; *mean_nll (host, may be NULL). */
 int piqmc_made_train_step(piqmc_made *m, int B, const int8_t *s_dev, float lr, double *mean_nll,
                           void *stream);
+/* One epoch of piqmc_made_train_step over n slices data_dev int8 [n][N] in
+ * the order perm_dev int32 [n]: floor(n / batch) Adam steps of `batch` rows,
+ * launched back to back on the stream with one host synchronisation at the
+ * end, for *mean_nll (host, may be NULL), the epoch's mean NLL. */
+int piqmc_made_train_epoch(piqmc_made *m, int n, const int8_t *data_dev, const int32_t *perm_dev, int batch,
+                           float lr, double *mean_nll, void *stream);
 /* B autoregressive draws -> int8 [B][N] and their log q (float64 [B]).
  * Draw d of row b uses philox(key=seed, ctr={round, b, d, 0x5A4D0000})[0].
  * The N-step launch chain runs as a captured HIP graph (replayed per call)
@@ -268,6 +286,35 @@ int piqmc_made_mh(piqmc_made *m, const piqmc_problem *pb, int R, int P, int8_t *
                   const int8_t *new_dev, const double *logq_new_dev, const int32_t *slice_dev,
                   float jperp, float temp, uint64_t seed, uint32_t round, int32_t *accepted_dev,
                   void *stream);
+/* The fused autoregressive sampler: one kernel pass per slice instead of N
+ * forward passes (running pre-activations, each unit added once when it
+ * completes).  It defines q_beta(s) = prod_d sigmoid(+-beta l_d(s_<d)), a
+ * normalized density for every beta > 0 (beta = 1: the network; beta > 1:
+ * sharpened towards its modes).  Rows [0, Bs): draws into s_out_dev int8
+ * [Bs][N]; rows [Bs, Bs + Bf): the given slices forced_dev int8 [Bf][N];
+ * logq_dev float64 [Bs + Bf] gets log q_beta of every row.  A draw and the
+ * evaluation of the drawn slice give the same log q bit for bit.  Draw d of
+ * row b uses philox(key=seed, ctr={round, b, d/4, 0x4D415200})[d % 4].
+ * Supported shapes: 2 <= N <= 65 with H <= 4(N-1), 66 <= N <= 129 with
+ * H <= 2(N-1), 194 <= N <= 257 with H <= 3(N-1), 1666 <= N <= 1729 with
+ * H < N (else PIQMC_EINVAL). */
+int piqmc_made_ar(piqmc_made *m, int Bs, int Bf, const int8_t *forced_dev, float beta, uint64_t seed,
+                  uint32_t round, int8_t *s_out_dev, double *logq_dev, void *stream);
+/* World-line Metropolis-Hastings move for R replicas of confs_dev int8
+ * [R][P][N]: y ~ q_beta, f = y * s^0 (s^0 the replica's slice 0), and every
+ * slice s^k becomes s^k * f, which keeps every Trotter bond; accepted with
+ * min(1, exp(-sum_k [E(s^k f) - E(s^k)] / T) q_beta(s^0) / q_beta(y)).  The
+ * stationary law of the sweep is unchanged.  accepted_dev int32 [R]. */
+int piqmc_made_worldline_mh(piqmc_made *m, const piqmc_problem *pb, int R, int P, int8_t *confs_dev,
+                            float beta, float temp, uint64_t seed, uint32_t round, int32_t *accepted_dev,
+                            void *stream);
+/* piqmc_made_mh with the proposal drawn by the fused sampler: replica r's
+ * slice k = slice_dev[r] is replaced by y ~ q_beta, accepted with
+ * min(1, exp(-dS/T) q_beta(old) / q_beta(y)) (dS as in piqmc_made_mh); both
+ * densities come from the fused sampler's arithmetic. */
+int piqmc_made_slice_mh(piqmc_made *m, const piqmc_problem *pb, int R, int P, int8_t *confs_dev,
+                        const int32_t *slice_dev, float beta, float jperp, float temp, uint64_t seed,
+                        uint32_t round, int32_t *accepted_dev, void *stream);
 /* C[M][N] = A[M][K] . B[N][K]^T, bf16 (uint16 bit patterns) in, fp32 out. */
 int piqmc_gemm_bf16_nt(const uint16_t *A, const uint16_t *B, int M, int N, int K, float *C,
                        void *stream);

@@ -80,6 +80,7 @@ struct ColArgs {
     const uint32_t *ztab;      // [S][8] per degree z: u_s thresholds (see run_colored)
     WT *packed;                // [R][2][N]
     double *etrace;            // [R][S] or null
+    double *fields;            // [R][2][W][N|1] local fields (cached-field dense sweep)
 };
 
 __device__ __forceinline__ int popc(uint32_t x) { return __popc(x); }
@@ -1545,6 +1546,224 @@ __global__ void __launch_bounds__(kDenseMaxBlock) dense_split_kernel(ColArgs<WT>
     dense_replica<WT, true>(a, r, smem + (size_t)w * dense_wave_bytes(a.N, a.Z, sizeof(WT)));
 }
 
+// ------------------------------------------------------------------ dense path, cached fields
+// For rows that sum exactly in any order and list each neighbour once, with
+// the same coupling in both directions (piqmc_problem's fields_ok), the local
+// field of every spin in every slice of the parity being swept stays in LDS:
+// a decision reads one field per slice instead of summing a row, and a flip
+// of spin i in slice m adds 2 J_ij s_i^new to the field of each neighbour j
+// in slice m.  Every field is a signed sum of its row's couplings, so with
+// exact rows each update lands on exactly the double a fresh row sum gives:
+// the decisions, Philox words and flips are those of the other dense bodies.
+// A parity's fields change only while that parity is swept, so the two
+// tables live in global memory and the active one is staged per half sweep.
+//
+// LDS of one replica: state [2][N] words, then (8-byte aligned) fields [W][NP]
+// doubles with NP = N rounded up to odd: a decision reads one field per slice
+// at a stride of NP doubles, an update writes consecutive spins of one slice,
+// and neither conflicts on the banks.
+__host__ __device__ inline int fields_pitch(int N) { return N | 1; }
+__host__ __device__ inline size_t dense_fields_bytes(int N, int W, int word_bytes)
+{
+    const size_t state = ((size_t)2 * N * word_bytes + 7) & ~(size_t)7;
+    return state + (size_t)W * fields_pitch(N) * sizeof(double);
+}
+
+// Decision of spin i in the W slices of one parity word, lane m < W = slice
+// 2m+tau, given the exact spatial field h of this lane's slice: the
+// arithmetic, classes and Philox words of the lane-per-slice body.  Returns
+// the flip mask (bit m = slice m flips), wave-uniform.
+template <typename WT>
+__device__ __forceinline__ WT dense_decide(const ColArgs<WT> &a, double h, WT S_, WT O, int lane, int tau,
+                                           float jp, double Ts, uint32_t g, int i, uint32_t rep)
+{
+    const int W = a.W;
+    const WT one = WT(1);
+    const bool active = lane < W;
+    const int lb = active ? lane : 0;
+    const int ml = tau == 0 ? (lb == 0 ? W - 1 : lb - 1) : lb;
+    const int mr = tau == 0 ? lb : (lb == W - 1 ? 0 : lb + 1);
+    const double si = ((S_ >> lb) & one) ? -1.0 : 1.0;
+    const double ts = (((O >> ml) & one) ? -1.0 : 1.0) + (((O >> mr) & one) ? -1.0 : 1.0);
+    const double ed = -2.0 * si * (h + (double)jp * ts);
+    // ed < -23 T: exp(ed/T) < 1.1e-10 < 2^-33, never accepted (class 0 of
+    // piqmc_fixed_class) -- decided without the division and the exponential
+    const bool pos = ed > 0.0, far = Ts > 0.0 && ed < -23.0 * Ts;
+    bool fl = active && pos;
+    if (__ballot(active && !pos && !far)) {
+        // the exponential and the first two Philox blocks are independent:
+        // computed together, their latencies overlap
+        constexpr int kPer = sizeof(WT) == 4 ? 4 : 2;
+        const uint32_t c3 = uint32_t(tau) << 24;
+        uint32_t o0[4], o1[4];
+        philox4x32_10(g, (uint32_t)i, rep, c3, a.key0, a.key1, o0);
+        philox4x32_10(g, (uint32_t)i, rep, c3 | 1u, a.key0, a.key1, o1);
+        unsigned int thr = 0;
+        const int cls = (!active || pos || far) ? 0 : piqmc_fixed_class(ed, Ts, &thr);
+        if (cls == 1) fl = true;
+        if (__ballot(cls == 2)) {
+            bool und = cls == 2, lt = false;
+            for (int qd = 0; qd < 32 / kPer && __ballot(und); ++qd) {
+                uint32_t out[4];
+                if (qd == 0) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) out[t] = o0[t];
+                } else if (qd == 1) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) out[t] = o1[t];
+                } else {
+                    philox4x32_10(g, (uint32_t)i, rep, c3 | (uint32_t)qd, a.key0, a.key1, out);
+                }
+#pragma unroll
+                for (int t = 0; t < kPer; ++t) {
+                    const uint32_t ub = (uint32_t)((rword<WT>(out, t) >> lb) & one);
+                    const uint32_t tb = (thr >> (31 - (qd * kPer + t))) & 1u;
+                    if (und && ub != tb) {
+                        lt = ub < tb;
+                        und = false;
+                    }
+                }
+            }
+            if (cls == 2) fl = lt;
+        }
+    }
+    return (WT)__ballot(fl);
+}
+
+template <typename WT>
+__device__ void dense_fields_replica(const ColArgs<WT> &a, int r, unsigned char *base)
+{
+    const int N = a.N, W = a.W, Z = a.Z, lane = threadIdx.x & 63, NP = fields_pitch(N);
+    WT *st = reinterpret_cast<WT *>(base);
+    double *tab = reinterpret_cast<double *>(base + (((size_t)2 * N * sizeof(WT) + 7) & ~(size_t)7));
+    const WT one = WT(1);
+    const bool active = lane < W;
+    const int lb = active ? lane : 0;
+    const uint32_t rep = a.replica_base + (uint32_t)r;
+    WT *gst = a.packed + (size_t)r * 2 * N;
+    double *gt = a.fields + (size_t)r * 2 * W * NP;  // [2][W][NP]
+    for (int q = lane; q < 2 * N; q += 64) st[q] = gst[q];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // fields of both parities from the rows (a field entry counts with +J)
+    for (int tau = 0; tau < 2; ++tau) {
+        const WT *w = st + tau * N;
+        for (int j = 0; j < N; ++j) {
+            double h = 0.0;
+            for (int e = 0; e < Z; ++e) {
+                const int2 rw = a.rows[(size_t)j * Z + e];
+                const double J = (double)__int_as_float(rw.y);
+                const WT wj = rw.x == j ? WT(0) : w[rw.x];
+                h += ((wj >> lb) & one) ? -J : J;
+            }
+            if (active) gt[((size_t)tau * W + lane) * NP + j] = h;
+        }
+    }
+    constexpr int kPre = 4;  // row entries per lane held in registers (rows up to 256)
+    int2 pre[kPre], cur[kPre];
+    auto fetch_row = [&](int i) {
+        const int2 *row = a.rows + (size_t)i * Z;
+#pragma unroll
+        for (int k = 0; k < kPre; ++k) pre[k] = row[min(lane + 64 * k, Z - 1)];
+    };
+    for (int s = 0; s < a.S; ++s) {
+        const float jp = a.jperp[s];
+        const double Ts = (double)a.temps[s];
+        for (int m = 0; m < a.M; ++m) {
+            const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
+            for (int tau = 0; tau < 2; ++tau) {
+                WT *mine = st + tau * N;
+                const WT *other = st + (1 - tau) * N;
+                double *gtt = gt + (size_t)tau * W * NP;
+                for (int t = lane; t < W * NP; t += 64) tab[t] = gtt[t];
+                fetch_row(__builtin_amdgcn_readfirstlane(a.order[0]));
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (int q = 0; q < N; ++q) {
+                    const int i = __builtin_amdgcn_readfirstlane(a.order[q]);
+#pragma unroll
+                    for (int k = 0; k < kPre; ++k) cur[k] = pre[k];
+                    if (q + 1 < N) fetch_row(__builtin_amdgcn_readfirstlane(a.order[q + 1]));
+                    const double h = tab[(size_t)lb * NP + i];
+                    const WT S_ = mine[i], O = other[i];
+                    const WT bal = dense_decide<WT>(a, h, S_, O, lane, tau, jp, Ts, g, i, rep);
+                    if (lane == 0) mine[i] = S_ ^ bal;
+                    // neighbours' fields in the slices where spin i flipped:
+                    // + 2 J s_i^new (old bit set = -1, so the new spin is +1).
+                    // Up to kFl flipped slices per pass: all their reads issue
+                    // before any add, so a pass costs one LDS round trip.
+                    constexpr int kFl = 4;
+                    int jj[kPre];
+                    double jv[kPre];
+#pragma unroll
+                    for (int k = 0; k < kPre; ++k) {
+                        const float J = __int_as_float(cur[k].y);
+                        const bool use = lane + 64 * k < Z && cur[k].x != i && J != 0.0f;
+                        jj[k] = use ? cur[k].x : -1;
+                        jv[k] = 2.0 * (double)J;
+                    }
+                    WT rem = bal;
+                    while (rem) {
+                        int mm[kFl];
+                        double sg[kFl];
+#pragma unroll
+                        for (int f = 0; f < kFl; ++f) {
+                            mm[f] = rem ? ctz(rem) : -1;
+                            sg[f] = mm[f] >= 0 && ((S_ >> mm[f]) & one) ? 1.0 : -1.0;
+                            rem &= rem - (rem ? WT(1) : WT(0));
+                        }
+                        double v[kFl][kPre];
+#pragma unroll
+                        for (int f = 0; f < kFl; ++f)
+#pragma unroll
+                            for (int k = 0; k < kPre; ++k)
+                                v[f][k] = (mm[f] >= 0 && jj[k] >= 0) ? tab[(size_t)mm[f] * NP + jj[k]] : 0.0;
+#pragma unroll
+                        for (int f = 0; f < kFl; ++f)
+#pragma unroll
+                            for (int k = 0; k < kPre; ++k)
+                                if (mm[f] >= 0 && jj[k] >= 0) tab[(size_t)mm[f] * NP + jj[k]] = v[f][k] + sg[f] * jv[k];
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+                for (int t = lane; t < W * NP; t += 64) gtt[t] = tab[t];
+            }
+        }
+        if (a.etrace) {
+            double acc = 0.0;
+            for (int i = lane; i < N; i += 64) {
+                const WT a0 = st[i], a1 = st[N + i];
+                const int2 *row = a.rows + (size_t)i * a.Z;
+                for (int e = 0; e < a.Z; ++e) {
+                    const int2 rw = row[e];
+                    const double J = (double)__int_as_float(rw.y);
+                    if (J == 0.0) continue;
+                    if (rw.x == i) acc -= J * (double)(a.P - 2 * (popc(a0) + popc(a1)));
+                    else acc -= 0.5 * J * (double)(a.P - 2 * (popc(a0 ^ st[rw.x]) + popc(a1 ^ st[N + rw.x])));
+                }
+            }
+            acc = wave_sum(acc);
+            if (lane == 0) a.etrace[(size_t)r * a.S + s] = acc / (double)a.P;
+        }
+    }
+    for (int q = lane; q < 2 * N; q += 64) gst[q] = st[q];
+}
+
+// One replica per wavefront; as many wavefronts per workgroup as the launch
+// gives (the host keeps the LDS of all of them within 160 KiB).
+template <typename WT>
+__global__ void dense_fields_kernel(ColArgs<WT> a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int w = threadIdx.x >> 6, r = blockIdx.x * (blockDim.x >> 6) + w;
+    if (r >= a.R) return;
+    dense_fields_replica<WT>(a, r, smem + (size_t)w * dense_fields_bytes(a.N, a.W, sizeof(WT)));
+}
+
 // int8 [R][P][N] -> packed [R][2][N] (bit m of parity tau = slice 2m+tau, set = -1)
 template <typename WT>
 __global__ void pack_parity_kernel(const int8_t *confs, WT *packed, int R, int P, int N)
@@ -1750,6 +1969,8 @@ size_t colored_scratch_bytes(const piqmc_problem *pb, int R, int P, int S)
 {
     const size_t wb = P / 2 <= 32 ? 4 : 8;
     size_t b = CallBuffers::round_up(sizeof(float) * (S ? 2 * S : 1));
+    if (use_dense(pb) && pb->dense_variant == 6)
+        b += CallBuffers::round_up(sizeof(double) * 2 * (size_t)(P / 2) * (pb->N | 1) * R);
     if (!use_dense(pb)) {
         if (pb->pmj && S)
             b += CallBuffers::round_up(sizeof(uint32_t) * (size_t)S * kClassStride) +
@@ -1779,20 +2000,50 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         if (block > 1024 || (block & 63)) return piqmc::set_error(PIQMC_EINVAL, "block must be a multiple of 64, <= 1024");
         const int wpb = block / 64;
         const size_t lds = dense_wave_bytes(N, pb->Z, sizeof(WT)) * wpb;
-        // a workgroup per replica while replicas are few (1.2-1.9x up to 1024
-        // 256-spin replicas); from 4096 on, a wavefront each is 1.3x faster
-        const bool wg0 = pb->exact_rows && dense_wg_enabled() && R < 2048;
-        const int pw = dense_pipe_waves(pb->Z, P / 2, R);
-        const bool pipe = wg0 && dense_pipe_enabled() && dense_pipe_bytes(N, pb->Z, sizeof(WT), pw - 1) <= 160 * 1024;
-        const size_t lwg = pipe ? dense_pipe_bytes(N, pb->Z, sizeof(WT), pw - 1) : dense_wg_bytes(N, pb->Z, sizeof(WT));
-        const bool wg = wg0 && lwg <= 160 * 1024;
-        if (!wg && lds > 160 * 1024) return piqmc::set_error(PIQMC_EINVAL, "dense sweep: state exceeds LDS");
+        const int W = P / 2;
+        const bool split_ok = pb->exact_rows && W <= 16 && 16 % W == 0 && block <= kDenseMaxBlock;
+        bool wg = false, pipe = false, split = false;
+        int pw = 4;
+        size_t lwg = 0;
+        if (pb->dense_variant == 0) {
+            // a workgroup per replica while replicas are few (1.2-1.9x up to 1024
+            // 256-spin replicas); from 4096 on, a wavefront each is 1.3x faster
+            const bool wg0 = pb->exact_rows && dense_wg_enabled() && R < 2048;
+            pw = dense_pipe_waves(pb->Z, W, R);
+            pipe = wg0 && dense_pipe_enabled() && dense_pipe_bytes(N, pb->Z, sizeof(WT), pw - 1) <= 160 * 1024;
+            lwg = pipe ? dense_pipe_bytes(N, pb->Z, sizeof(WT), pw - 1) : dense_wg_bytes(N, pb->Z, sizeof(WT));
+            wg = wg0 && lwg <= 160 * 1024;
+            split = !wg && split_ok;
+        } else if (pb->dense_variant == 2) {
+            if (!split_ok)
+                return piqmc::set_error(PIQMC_EINVAL, "dense variant 2 (slice split over lanes) needs exactly summable "
+                                                      "rows, P/2 dividing 16 and block <= 256");
+            split = true;
+        } else if (pb->dense_variant == 6) {
+            if (!pb->exact_rows || !pb->fields_ok || dense_fields_bytes(N, W, sizeof(WT)) > 160 * 1024)
+                return piqmc::set_error(PIQMC_EINVAL, "dense variant 6 (cached fields) needs exactly summable rows "
+                                                      "that list each neighbour once with the same coupling both "
+                                                      "ways, and W x N fields that fit the LDS");
+        } else if (pb->dense_variant >= 3) {
+            pipe = pb->dense_variant >= 4;
+            pw = pb->dense_variant == 5 ? 8 : 4;
+            lwg = pipe ? dense_pipe_bytes(N, pb->Z, sizeof(WT), pw - 1) : dense_wg_bytes(N, pb->Z, sizeof(WT));
+            if (!pb->exact_rows || lwg > 160 * 1024)
+                return piqmc::set_error(PIQMC_EINVAL, "dense variants 3-5 (a workgroup per replica) need exactly "
+                                                      "summable rows and a row that fits the LDS");
+            wg = true;
+        }
+        const bool fields = pb->dense_variant == 6;
+        if (!wg && !fields && lds > 160 * 1024) return piqmc::set_error(PIQMC_EINVAL, "dense sweep: state exceeds LDS");
         float *d_jp = nullptr;  // [S] J_perp, then [S] temperatures
         WT *d_packed = nullptr;
+        double *d_fields = nullptr;
         if ((rc = buf.take((void **)&d_jp, sizeof(float) * (S ? 2 * S : 1)))) return rc;
         if (S) PIQMC_HIP(hipMemcpyAsync(d_jp, jp.data(), sizeof(float) * S, hipMemcpyHostToDevice, st));
         if (S) PIQMC_HIP(hipMemcpyAsync(d_jp + S, temps.data(), sizeof(float) * S, hipMemcpyHostToDevice, st));
         if ((rc = buf.take((void **)&d_packed, sizeof(WT) * 2 * (size_t)N * R))) return rc;
+        if (fields && (rc = buf.take((void **)&d_fields, sizeof(double) * 2 * (size_t)W * fields_pitch(N) * R)))
+            return rc;
         launch_pack<WT>(confs, d_packed, R, P, N, st);
         ColArgs<WT> a = {};
         a.N = N; a.Z = pb->Z; a.P = P; a.W = P / 2; a.S = S; a.M = M; a.R = R;
@@ -1800,10 +2051,16 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
         a.replica_base = rbase; a.sweep_base = sbase;
         a.jperp = d_jp; a.rows = pb->d_rows; a.order = pb->d_order; a.prevj = pb->d_prevj;
-        a.packed = d_packed; a.etrace = etrace;
+        a.packed = d_packed; a.etrace = etrace; a.fields = d_fields;
         piqmc::timing_begin(st);
-        const int W = P / 2;
-        if (wg && pipe) {
+        if (fields) {
+            // one replica per wavefront, as many per workgroup as block and LDS allow
+            const size_t per = dense_fields_bytes(N, W, sizeof(WT));
+            const int wf = std::max(1, std::min(wpb, (int)((160 * 1024) / per)));
+            (void)hipFuncSetAttribute((const void *)dense_fields_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(per * wf));
+            dense_fields_kernel<WT><<<(R + wf - 1) / wf, 64 * wf, per * wf, st>>>(a);
+        } else if (wg && pipe) {
             // one replica per workgroup, one spin decided while the next is summed
             (void)hipFuncSetAttribute((const void *)dense_pipe_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lwg);
             dense_pipe_kernel<WT><<<R, 64 * pw, lwg, st>>>(a);
@@ -1811,7 +2068,7 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
             // one replica per workgroup (the field split over four wavefronts)
             (void)hipFuncSetAttribute((const void *)dense_wg_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lwg);
             dense_wg_kernel<WT><<<R, 256, lwg, st>>>(a);
-        } else if (pb->exact_rows && W <= 16 && 16 % W == 0 && block <= kDenseMaxBlock) {
+        } else if (split) {
             // split when it gives at least four lanes per slice (P = 16: 2.2x,
             // P = 32: 1.4x on 256-spin SK; two lanes per slice measured 4 % slower)
             (void)hipFuncSetAttribute((const void *)dense_split_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -12,6 +12,8 @@
 
 #include <climits>
 
+#include <tuple>
+#include <cstring>
 #include "piqmc_internal.h"
 
 namespace piqmc {
@@ -369,6 +371,30 @@ extern "C" int piqmc_problem_create(int N, int Z, const double *nbs, piqmc_probl
         (void)frexp(bound, &eb);                    // every partial sum is below 2^eb
         if (eb - lsb > 52) p->exact_rows = 0;
     }
+    {
+        // fields_ok: every nonzero coupling (i, j != i, J) is the row's only
+        // entry for j, and (j, i, J) is in row j
+        std::vector<std::tuple<int, int, uint32_t>> ent;
+        for (int i = 0; i < N; ++i)
+            for (int e = 0; e < Z; ++e) {
+                const float Jv = p->J[(size_t)i * Z + e];
+                const int j = p->idx[(size_t)i * Z + e];
+                if (Jv != 0.0f && j != i) {
+                    uint32_t b;
+                    std::memcpy(&b, &Jv, 4);
+                    ent.emplace_back(i, j, b);
+                }
+            }
+        std::sort(ent.begin(), ent.end());
+        bool ok = true;
+        for (size_t t = 1; t < ent.size() && ok; ++t)
+            if (std::get<0>(ent[t]) == std::get<0>(ent[t - 1]) && std::get<1>(ent[t]) == std::get<1>(ent[t - 1]))
+                ok = false;
+        for (size_t t = 0; t < ent.size() && ok; ++t)
+            ok = std::binary_search(ent.begin(), ent.end(),
+                                    std::make_tuple(std::get<1>(ent[t]), std::get<0>(ent[t]), std::get<2>(ent[t])));
+        p->fields_ok = ok ? 1 : 0;
+    }
     if (p->zmax > 7) p->pmj = p->pmjf = 0;  // bit-sliced counter holds 0..7
     if (p->pmj || nfield == 0) p->pmjf = 0;
     p->zuni = p->zmax;
@@ -499,6 +525,13 @@ extern "C" int piqmc_problem_set_sweep_path(piqmc_problem *p, int path)
 {
     if (!p || path < 0 || path > 2) return set_error(PIQMC_EINVAL, "sweep path must be 0 (auto), 1 or 2");
     p->sweep_path = path;
+    return PIQMC_OK;
+}
+
+extern "C" int piqmc_problem_set_dense_variant(piqmc_problem *p, int variant)
+{
+    if (!p || variant < 0 || variant > 6) return set_error(PIQMC_EINVAL, "dense variant must be 0 (auto) .. 6");
+    p->dense_variant = variant;
     return PIQMC_OK;
 }
 

@@ -22,6 +22,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "piqmc_internal.h"
 
@@ -368,30 +369,7 @@ __global__ void sample_step(const float *logits, int B, int N, int d, const uint
 
 }  // namespace
 
-// ------------------------------------------------------------------ MADE state
-struct piqmc_made {
-    int N = 0, H1 = 0, H2 = 0, cap = 0;  // cap = max batch rows of the buffers
-    int layers_in[3], layers_out[3];
-    float *w[3] = {}, *b[3] = {}, *mw[3] = {}, *vw[3] = {}, *mb[3] = {}, *vb[3] = {};
-    uint16_t *wb[3] = {}, *wt[3] = {};
-    uint8_t *mask[3] = {};
-    float *gw[3] = {}, *gb[3] = {};
-    // activations [cap][*]
-    uint16_t *x = nullptr, *h1 = nullptr, *h2 = nullptr;
-    float *logits = nullptr;
-    uint16_t *dlog = nullptr, *dh2 = nullptr, *dh1 = nullptr;
-    uint16_t *tA = nullptr, *tB = nullptr;  // transposes for weight gradients
-    double *nll = nullptr;
-    int8_t *s = nullptr;
-    int step = 0;
-    // sampling graph (captured once per batch size on a private stream)
-    uint32_t *rng = nullptr;  // {key0, key1, round}
-    hipStream_t gs = nullptr;
-    hipEvent_t ev_in = nullptr, ev_out = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int gB = 0;
-    int use_graph = 1;
-};
+#include "made_state.h"
 
 namespace {
 int made_alloc_batch(piqmc_made *m, int B)
@@ -506,13 +484,15 @@ extern "C" int piqmc_made_destroy(piqmc_made *m)
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
-    void *bs[] = {m->x, m->h1, m->h2, m->logits, m->dlog, m->dh2, m->dh1, m->tA, m->tB, m->nll, m->s, m->rng};
+    void *bs[] = {m->x, m->h1, m->h2, m->logits, m->dlog, m->dh2, m->dh1, m->tA, m->tB, m->nll, m->s, m->rng,
+                  m->epoch_rows, m->nll_acc};
     for (void *p : bs)
         if (p) (void)hipFree(p);
     if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
     if (m->ev_in) (void)hipEventDestroy(m->ev_in);
     if (m->ev_out) (void)hipEventDestroy(m->ev_out);
     if (m->gs) (void)hipStreamDestroy(m->gs);
+    piqmc::made_ar_release(m);
     delete m;
     return PIQMC_OK;
 }
@@ -562,13 +542,12 @@ extern "C" int piqmc_made_logprob(piqmc_made *m, int B, const int8_t *s_dev, dou
 
 // Gradients of the mean NLL over B slices into the gradient buffers; returns
 // the mean NLL.  With lr > 0 also applies one Adam step.
-extern "C" int piqmc_made_train_step(piqmc_made *m, int B, const int8_t *s_dev, float lr, double *mean_nll,
-                                     void *stream)
+namespace {
+// One training step on the stream, no host synchronisation: the gradients
+// of the mean NLL of B slices, and an Adam step when lr > 0.  Per-row NLL in
+// m->nll.
+int train_core(piqmc_made *m, int B, const int8_t *s_dev, float lr, hipStream_t st)
 {
-    if (!m || B < 1 || !s_dev) return piqmc::set_error(PIQMC_EINVAL, "made_train_step: bad args");
-    int rc = made_alloc_batch(m, B);
-    if (rc) return rc;
-    hipStream_t st = (hipStream_t)stream;
     const int N = m->N, H1 = m->H1, H2 = m->H2;
     spins_to_bf16<<<grid1((size_t)B * N), 256, 0, st>>>(s_dev, (size_t)B * N, m->x);
     forward(m, B, st);
@@ -594,6 +573,7 @@ extern "C" int piqmc_made_train_step(piqmc_made *m, int B, const int8_t *s_dev, 
     col_sum_bf16<<<(H1 + 255) / 256, 256, 0, st>>>(m->dh1, B, H1, m->gb[0]);
     if (lr > 0.f) {
         m->step++;
+        m->ar_dirty = 1;
         const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
         const float c1 = 1.f - powf(b1, (float)m->step), c2 = 1.f - powf(b2, (float)m->step);
         for (int l = 0; l < 3; ++l) {
@@ -604,6 +584,47 @@ extern "C" int piqmc_made_train_step(piqmc_made *m, int B, const int8_t *s_dev, 
         }
     }
     PIQMC_HIP(hipGetLastError());
+    return PIQMC_OK;
+}
+
+// rows data[perm[b0 + b]] -> out [B][N]
+__global__ void gather_rows(const int8_t *data, const int32_t *perm, int b0, int B, int N, int8_t *out)
+{
+    const size_t tot = (size_t)B * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
+        const int b = (int)(t / N), i = (int)(t % N);
+        out[t] = data[(size_t)perm[b0 + b] * N + i];
+    }
+}
+
+// acc[0] += sum of v[0..n) (one workgroup; launches on one stream run in order)
+__global__ void accumulate_sum(const double *v, int n, double *acc)
+{
+    __shared__ double red[16];
+    double t = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) t += v[i];
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) s += red[w];
+        acc[0] += s;
+    }
+}
+
+__global__ void zero_double(double *p) { *p = 0.0; }
+}  // namespace
+
+extern "C" int piqmc_made_train_step(piqmc_made *m, int B, const int8_t *s_dev, float lr, double *mean_nll,
+                                     void *stream)
+{
+    if (!m || B < 1 || !s_dev) return piqmc::set_error(PIQMC_EINVAL, "made_train_step: bad args");
+    int rc = made_alloc_batch(m, B);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    rc = train_core(m, B, s_dev, lr, st);
+    if (rc) return rc;
     if (mean_nll) {
         std::vector<double> h(B);
         PIQMC_HIP(hipMemcpyAsync(h.data(), m->nll, 8 * (size_t)B, hipMemcpyDeviceToHost, st));
@@ -611,6 +632,45 @@ extern "C" int piqmc_made_train_step(piqmc_made *m, int B, const int8_t *s_dev, 
         double t = 0;
         for (double v : h) t += v;
         *mean_nll = t / B;
+    }
+    return PIQMC_OK;
+}
+
+// One epoch of Adam over n slices data_dev int8 [n][N] in the order perm_dev
+// int32 [n]: floor(n / batch) steps of `batch` rows, launched back to back
+// with no host synchronisation until the epoch's mean NLL is read at the end.
+extern "C" int piqmc_made_train_epoch(piqmc_made *m, int n, const int8_t *data_dev, const int32_t *perm_dev,
+                                      int batch, float lr, double *mean_nll, void *stream)
+{
+    if (!m || n < 1 || batch < 1 || batch > n || !data_dev || !perm_dev)
+        return piqmc::set_error(PIQMC_EINVAL, "made_train_epoch: bad args");
+    // the batch rows live past the sampler's rows in a buffer of its own
+    int rc = made_alloc_batch(m, batch);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    if (!m->epoch_rows || m->epoch_cap < batch) {
+        if (m->epoch_rows) PIQMC_HIP(hipFree(m->epoch_rows));
+        m->epoch_rows = nullptr;
+        m->epoch_cap = 0;
+        PIQMC_HIP(hipMalloc(&m->epoch_rows, (size_t)batch * m->N));
+        m->epoch_cap = batch;
+    }
+    if (!m->nll_acc) PIQMC_HIP(hipMalloc(&m->nll_acc, sizeof(double)));
+    zero_double<<<1, 1, 0, st>>>(m->nll_acc);
+    const int steps = n / batch;
+    for (int t = 0; t < steps; ++t) {
+        gather_rows<<<grid1((size_t)batch * m->N), 256, 0, st>>>(data_dev, perm_dev, t * batch, batch, m->N,
+                                                                 m->epoch_rows);
+        rc = train_core(m, batch, m->epoch_rows, lr, st);
+        if (rc) return rc;
+        accumulate_sum<<<1, 256, 0, st>>>(m->nll, batch, m->nll_acc);
+    }
+    PIQMC_HIP(hipGetLastError());
+    if (mean_nll) {
+        double h = 0.0;
+        PIQMC_HIP(hipMemcpyAsync(&h, m->nll_acc, sizeof(double), hipMemcpyDeviceToHost, st));
+        PIQMC_HIP(hipStreamSynchronize(st));
+        *mean_nll = h / ((double)steps * batch);
     }
     return PIQMC_OK;
 }
@@ -760,6 +820,32 @@ extern "C" int piqmc_made_mh(piqmc_made *m, const piqmc_problem *pb, int R, int 
                                  (uint32_t)seed, (uint32_t)(seed >> 32), round, accepted);
     PIQMC_HIP(hipFreeAsync(old, st));
     PIQMC_HIP(hipFreeAsync(lq_old, st));
+    PIQMC_HIP(hipGetLastError());
+    return PIQMC_OK;
+}
+
+// The independence slice move with the fused sampler: slice k = slice_dev[r]
+// of replica r is replaced by y ~ q_beta with min(1, exp(-dS/T) q_beta(old) /
+// q_beta(y)); both densities come from the fused sampler's arithmetic.
+extern "C" int piqmc_made_slice_mh(piqmc_made *m, const piqmc_problem *pb, int R, int P, int8_t *confs,
+                                   const int32_t *slice, float beta, float jperp, float temp, uint64_t seed,
+                                   uint32_t round, int32_t *accepted, void *stream)
+{
+    if (!m || !pb || R < 1 || P < 1 || !confs || !slice || !accepted)
+        return piqmc::set_error(PIQMC_EINVAL, "made_slice_mh: bad args");
+    if (pb->N != m->N) return piqmc::set_error(PIQMC_EINVAL, "made_slice_mh: network and problem sizes differ");
+    if (!(beta > 0.f) || !std::isfinite(beta)) return piqmc::set_error(PIQMC_EINVAL, "made_slice_mh: beta must be > 0");
+    hipStream_t st = (hipStream_t)stream;
+    int8_t *rows = nullptr;
+    double *lq = nullptr;
+    int rc = piqmc::made_ar_scratch(m, R, &rows, &lq);
+    if (rc) return rc;
+    int8_t *y = rows, *old = rows + (size_t)R * m->N;
+    gather_slices<<<grid1((size_t)R * m->N), 256, 0, st>>>(confs, R, P, m->N, slice, old);
+    rc = piqmc::made_ar_launch(m, R, R, old, beta, seed, round, y, lq, st);
+    if (rc) return rc;
+    mh_kernel<<<R, 256, 0, st>>>(confs, R, P, m->N, pb->Z, pb->d_rows, y, lq, lq + R, slice, jperp, temp,
+                                 (uint32_t)seed, (uint32_t)(seed >> 32), round, accepted);
     PIQMC_HIP(hipGetLastError());
     return PIQMC_OK;
 }

@@ -49,6 +49,7 @@ struct piqmc_problem {
     int zmax = 0;                 // max nonzero off-diagonal entries per row
     int zuni = -1;                // common degree when every row has zmax entries, else -1
     int sweep_path = 0;           // parallel sweep: 0 auto, 1 word per thread, 2 dense (lane per slice)
+    int dense_variant = 0;        // dense sweep layout: 0 auto, 1..5 (piqmc_problem_set_dense_variant)
     int32_t *d_cnb = nullptr;     // [zmax][N] neighbour of position q (color order)
     uint32_t *d_cinfo = nullptr;  // [N] bits 0..7 negative-coupling mask, 8..11 z
     // +-1 couplings with fields: as above, plus at most one field per spin with
@@ -60,6 +61,10 @@ struct piqmc_problem {
     // with E - L <= 52): the lane-per-slice sweep may then split and
     // reassociate the sum without changing a bit.
     int exact_rows = 0;
+    // Every row lists each neighbour at most once with a nonzero coupling, and
+    // that coupling appears, equal, in the neighbour's row: a flip of spin i
+    // then changes exactly the fields its own row names (cached-field sweep).
+    int fields_ok = 0;
     // [N] sum of the couplings of spin order[q] to spin order[q-1] (0 at q = 0):
     // the pipelined dense sweep adds that neighbour's term after its decision
     double *d_prevj = nullptr;

@@ -33,6 +33,7 @@ _SIGS = {
     "piqmc_problem_set_coloring": ([_vp, _i, _vp], _i),
     "piqmc_problem_get_coloring": ([_vp, _vp], _i),
     "piqmc_problem_set_sweep_path": ([_vp, _i], _i),
+    "piqmc_problem_set_dense_variant": ([_vp, _i], _i),
     "piqmc_rand_seed": ([_u32, _vp], _i),
     "piqmc_rand_advance": ([_vp, _u64], _i),
     "piqmc_perm_streams": ([_vp, _i, _i, _i, _vp, _i], _i),
@@ -59,9 +60,13 @@ _SIGS = {
     "piqmc_made_get_grads": ([_vp] * 7, _i),
     "piqmc_made_logprob": ([_vp, _i, _vp, _vp, _vp], _i),
     "piqmc_made_train_step": ([_vp, _i, _vp, _f, _vp, _vp], _i),
+    "piqmc_made_train_epoch": ([_vp, _i, _vp, _vp, _i, _f, _vp, _vp], _i),
     "piqmc_made_sample": ([_vp, _i, _u64, _u32, _vp, _vp, _vp], _i),
     "piqmc_made_set_graph": ([_vp, _i], _i),
     "piqmc_made_mh": ([_vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _f, _f, _u64, _u32, _vp, _vp], _i),
+    "piqmc_made_ar": ([_vp, _i, _i, _vp, _f, _u64, _u32, _vp, _vp, _vp], _i),
+    "piqmc_made_worldline_mh": ([_vp, _vp, _i, _i, _vp, _f, _f, _u64, _u32, _vp, _vp], _i),
+    "piqmc_made_slice_mh": ([_vp, _vp, _i, _i, _vp, _vp, _f, _f, _f, _u64, _u32, _vp, _vp], _i),
     "piqmc_gemm_bf16_nt": ([_vp, _vp, _i, _i, _i, _vp, _vp], _i),
     "piqmc_batch_create": ([C.POINTER(_vp)], _i),
     "piqmc_batch_destroy": ([_vp], _i),

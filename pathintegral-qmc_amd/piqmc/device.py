@@ -133,6 +133,18 @@ class Problem:
         code = {"auto": 0, "word": 1, "dense": 2}[path]
         _lib.call("piqmc_problem_set_sweep_path", self._h, code)
 
+    DENSE_VARIANTS = ("auto", "wave", "split", "workgroup", "pipe4", "pipe8", "fields")
+
+    def set_dense_variant(self, variant):
+        """Layout of the dense sweep: 'auto', 'wave' (a replica per wavefront,
+        a lane per slice), 'split' (a slice's field over several lanes),
+        'workgroup' (a replica per workgroup), 'pipe4' / 'pipe8' (that
+        workgroup pipelined with three / seven summing wavefronts), 'fields'
+        (every local field cached in LDS and updated on flips)."""
+        code = self.DENSE_VARIANTS.index(variant)
+        _lib.call("piqmc_problem_set_dense_variant", self._h, code)
+        self.dense_variant = variant
+
     def set_coloring(self, color):
         color = np.ascontiguousarray(color, np.int32)
         nc = int(color.max()) + 1
@@ -230,6 +242,7 @@ class Batch:
         _lib.call("piqmc_batch_create", C.byref(h))
         self._h = h
         self._keep = []
+        self.jobs = []
 
     def __del__(self):
         try:
@@ -250,9 +263,17 @@ class Batch:
                   sched.ctypes.data, confs.data_ptr(), int(seed) & 0xFFFFFFFFFFFFFFFF,
                   int(replica_base), int(sweep_base), et.data_ptr() if trace else None)
         self._keep.append((problem, confs, et))
+        self.jobs.append({"problem": problem, "R": R, "P": P, "sched": sched, "mcsteps": int(mcsteps),
+                          "temp": float(temp), "seed": int(seed), "replica_base": int(replica_base),
+                          "sweep_base": int(sweep_base)})
         return et
 
-    def run(self, block=0, rpw=0, stream=None):
+    def run(self, block=0, rpw=0, stream=None, tune=False):
+        """One launch over every job.  With ``tune`` and rpw == 0, the
+        replicas per work item come from the autotuner (piqmc.autotune.tune_batch)."""
+        if tune and not rpw:
+            from . import autotune
+            rpw = autotune.tune_batch(self)
         _lib.call("piqmc_batch_run", self._h, int(block), int(rpw), _stream(stream))
 
     def prepare(self, block=0, rpw=0, stream=None):

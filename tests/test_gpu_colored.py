@@ -367,3 +367,90 @@ def test_colored_pack_roundtrip(gpu, L, dim, P, offset):
     pb.qa_colored(c8, np.zeros(0), 1, 0.1, 1)
     torch.cuda.synchronize()
     assert np.array_equal(c8.cpu().numpy(), c0)
+
+
+# ------------------------------------------------------------ autotuned tiles
+@pytest.mark.parametrize("kind", ["pm", "sk"])
+def test_autotuned_tile_gives_the_oracle_spins(gpu, kind):
+    # the tile the autotuner picks (word path: threads and replicas per
+    # workgroup; dense path: the layout of the slices over lanes and
+    # wavefronts) sweeps to the CPU twin's spins
+    import torch
+    from piqmc import autotune, device, instances
+    if kind == "pm":
+        N, nbs, _, _, _ = pm_j_problem(8, seed=31)
+        P, R = 16, 6
+    else:
+        n, r, c, v = instances.sk(64, np.random.RandomState(32))
+        nbs = instances.neighbors(n, r, c, v)
+        N, P, R = n, 16, 4
+    pb = device.Problem(nbs)
+    sched = np.linspace(3.0, 1e-8, 20)
+    T = 1.0 / P
+    block, rpw = autotune.tune_colored(pb, P, R, T, sched, use_cache=False)
+    assert (kind == "sk") == autotune.is_dense(pb)
+    rng = np.random.RandomState(5)
+    c0 = (2 * rng.randint(2, size=(R, P, N)) - 1).astype(np.int8)
+    c8 = torch.tensor(c0, device=gpu)
+    pb.qa_colored(c8, sched, 1, T, 77, block=block, rpw=rpw)
+    torch.cuda.synchronize()
+    color = pb.coloring()
+    for k in range(R):
+        sp = c0[k].copy()
+        O.qa_colored(sched, 1, P, T, sp, nbs, color, int(color.max()) + 1, 77, replica=k)
+        assert np.array_equal(c8[k].cpu().numpy(), sp), (kind, block, rpw, k)
+
+
+def test_dense_variants_agree_with_the_oracle(gpu):
+    # every dense layout the tuner may set gives the CPU twin's spins
+    import torch
+    from piqmc import device, instances
+    n, r, c, v = instances.sk(48, np.random.RandomState(9))
+    nbs = instances.neighbors(n, r, c, v)
+    pb = device.Problem(nbs)
+    P, R, T = 16, 3, 1 / 16
+    sched = np.linspace(3.0, 1e-8, 12)
+    rng = np.random.RandomState(6)
+    c0 = (2 * rng.randint(2, size=(R, P, n)) - 1).astype(np.int8)
+    color = pb.coloring()
+    ref = []
+    for k in range(R):
+        sp = c0[k].copy()
+        O.qa_colored(sched, 1, P, T, sp, nbs, color, int(color.max()) + 1, 41, replica=k)
+        ref.append(sp)
+    for variant in device.Problem.DENSE_VARIANTS:
+        pb.set_dense_variant(variant)
+        c8 = torch.tensor(c0, device=gpu)
+        pb.qa_colored(c8, sched, 1, T, 41)
+        torch.cuda.synchronize()
+        assert np.array_equal(c8.cpu().numpy(), np.array(ref)), variant
+
+
+def test_autotuned_batch_equals_individual_runs(gpu):
+    # the replicas per work item tune_batch picks for a heterogeneous batch
+    # leave every job equal to its own qa_colored call
+    import torch
+    from piqmc import autotune, device, instances
+    rng = np.random.RandomState(3)
+    specs = [("pm", 16, 32, 30, 6), ("pm", 8, 16, 50, 5), ("sk", 64, 16, 12, 4), ("pm", 32, 64, 20, 3)]
+    b, jobs = device.Batch(), []
+    for t, (kind, size, P, S, R) in enumerate(specs):
+        if kind == "sk":
+            n, r, c, v = instances.sk(size, np.random.RandomState(t))
+        else:
+            n, r, c, v = instances.lattice_pm_j(size, 2, np.random.RandomState(t))
+        pb = device.Problem(instances.neighbors(n, r, c, v))
+        c0 = (2 * rng.randint(2, size=(R, P, n)) - 1).astype(np.int8)
+        c8 = torch.tensor(c0, device=gpu)
+        sched = np.linspace(3.0, 1e-8, S)
+        b.add(pb, c8, sched, 1, 1.0 / P, 500 + t, replica_base=7 * t)
+        jobs.append((pb, c0, c8, sched, P))
+    rpw = autotune.tune_batch(b, use_cache=False)
+    assert rpw in (1, 2, 4)
+    b.run(rpw=rpw)
+    torch.cuda.synchronize()
+    for t, (pb, c0, c8, sched, P) in enumerate(jobs):
+        ref = torch.tensor(c0, device=gpu)
+        pb.qa_colored(ref, sched, 1, 1.0 / P, 500 + t, replica_base=7 * t)
+        torch.cuda.synchronize()
+        assert torch.equal(c8, ref), t

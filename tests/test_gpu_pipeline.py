@@ -100,3 +100,25 @@ def test_pipeline_on_the_example_instance(gpu):
     gs = -1.55460615142578
     assert best.min() >= gs - 1e-7          # the listed value has 12 digits
     assert best.min() <= 0.97 * gs
+
+
+def test_pipeline_on_the_santoro_instance(gpu):
+    # the 80x80 Gaussian instance of examples/santoro80.py with its parameters
+    # (P=20, T=0.01, 100 field steps 1.5 -> 1e-8, preanneal 3.0 -> 0.01 over
+    # 100 steps), 64 replicas: no replica below the exact ground state
+    # (santoro_80x80_answer.txt:24), the best within 4 % of it, and every
+    # replica's best slice energy equal to the oracle's energy of that slice
+    from piqmc import device, instances, pipeline
+    N, rows, cols, vals = instances.read_instance(os.path.join(GOLDEN, "santoro_80x80.txt"))
+    assert N == 6400
+    nbs = instances.neighbors(N, rows, cols, vals)
+    pb = device.Problem(nbs)
+    res = pipeline.anneal(pb, 64, 20, 0.01, np.linspace(1.5, 1e-8, 100),
+                          pre_sched=np.linspace(3.0, 0.01, 100), seed=3)
+    best = res.best_energy.cpu().numpy()
+    gs = -1.58051667679
+    print("santoro best E/N %.6f, mean %.6f (ground state %.6f)" % (best.min() / N, best.mean() / N, gs))
+    assert best.min() / N >= gs - 1e-9
+    assert best.min() / N <= 0.96 * gs
+    E_ref = O.energy_nbs(res.best_conf.cpu().numpy(), nbs)
+    np.testing.assert_allclose(best, E_ref, rtol=0, atol=1e-8)

@@ -241,6 +241,29 @@ def test_instance_ground_state_energy():
     assert abs(E / 1024 - (-1.55460615142578)) < 1e-7
 
 
+def test_pmj_hit_energies_fixture():
+    # tests/golden/pmj_hit_energies.json: best-known energies of the +-J
+    # workload instances, each with the configuration that reached it; the
+    # oracle's energy of that configuration is the stored energy and no single
+    # flip lowers it (a +-J lattice energy is an even integer here)
+    import json
+    from piqmc import instances
+    with open(os.path.join(GOLDEN, "pmj_hit_energies.json")) as f:
+        db = json.load(f)
+    assert set(db) == {"L8_seed1", "L16_seed1", "L32_seed2024"}
+    for name, d in db.items():
+        N, r, c, v = instances.lattice_pm_j(d["L"], 2, np.random.RandomState(d["instance_seed"]))
+        nbs = instances.neighbors(N, r, c, v)
+        s = -np.ones(N, np.int8)
+        s[np.array(d["config_up_1based"]) - 1] = 1
+        E = float(O.energy_nbs(s[None, :].copy(), nbs)[0])
+        assert E == d["energy"], name
+        h = np.zeros(N)
+        np.add.at(h, r, v * s[c])
+        np.add.at(h, c, v * s[r])
+        assert np.all(s * h >= 0), name
+
+
 def test_santoro_80x80_ground_state_energy():
     # examples/ising_instances/santoro_80x80_answer.txt:24,36-256 -- exact GS of
     # the 80x80 Gaussian instance, read 1-based (the indices list the +1
