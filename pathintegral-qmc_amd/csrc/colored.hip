@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -108,6 +109,21 @@ __device__ T block_sum(T v, T *red)
     if (threadIdx.x == 0)
         for (int w = 0; w < nw; ++w) t += red[w];
     return t;
+}
+
+// Three-input XOR and majority of bit-sliced words (one gfx950 v_bitop3_b32
+// each for 32-bit words: truth tables 0x96 and 0xE8, symmetric in the inputs).
+template <typename WT>
+__device__ __forceinline__ WT xor3(WT a, WT b, WT c)
+{
+    if constexpr (sizeof(WT) == 4) return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+    else return a ^ b ^ c;
+}
+template <typename WT>
+__device__ __forceinline__ WT maj3(WT a, WT b, WT c)
+{
+    if constexpr (sizeof(WT) == 4) return __builtin_amdgcn_bitop3_b32(a, b, c, 0xE8);
+    else return (a & b) | (c & (a ^ b));
 }
 
 // t-th bit-sliced random word of a Philox4x32 output block: four 32-bit words
@@ -445,6 +461,29 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 WT flips = 0;
                                 if (PMJ) {
                                     WT c0 = 0, c1 = 0, c2 = 0;  // bit-sliced unsatisfied-bond counter
+                                    if constexpr (ZU == 4 || ZU == 6) {
+                                        // carry-save adder tree: a full adder is one XOR3 and one
+                                        // majority (v_bitop3 each), 6 logic ops for four bonds and
+                                        // 8 for six instead of 5 per bond for a ripple counter
+                                        WT x[ZU];
+    #pragma unroll
+                                        for (int z = 0; z < ZU; ++z)
+                                            x[z] = xor3<WT>(S_, mine[nb[z]], ((info >> z) & 1u) ? ~WT(0) : WT(0));
+                                        if constexpr (ZU == 4) {
+                                            const WT s1 = xor3<WT>(x[0], x[1], x[2]), k1 = maj3<WT>(x[0], x[1], x[2]);
+                                            c0 = s1 ^ x[3];
+                                            const WT k2 = s1 & x[3];
+                                            c1 = k1 ^ k2;
+                                            c2 = k1 & k2;
+                                        } else {
+                                            const WT s1 = xor3<WT>(x[0], x[1], x[2]), k1 = maj3<WT>(x[0], x[1], x[2]);
+                                            const WT s2 = xor3<WT>(x[3], x[4], x[5]), k2 = maj3<WT>(x[3], x[4], x[5]);
+                                            c0 = s1 ^ s2;
+                                            const WT k3 = s1 & s2;
+                                            c1 = xor3<WT>(k1, k2, k3);
+                                            c2 = maj3<WT>(k1, k2, k3);
+                                        }
+                                    } else {
     #pragma unroll
                                     for (int z = 0; z < 8; ++z) {
                                         if (ZU > 0 ? z < ZU : (z < a.zmax && z < zi)) {
@@ -455,6 +494,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                             c1 ^= k0;
                                             c2 ^= k1;
                                         }
+                                    }
                                     }
                                     // Trotter classes; bits past W are zero in every class
                                     const WT e0 = ~(uL | uR) & valid, e1 = uL ^ uR, e2 = uL & uR;
@@ -894,6 +934,71 @@ __host__ __device__ inline size_t dense_wave_bytes(int N, int Z, int word_bytes)
 {
     const size_t state = ((size_t)2 * N * word_bytes + 7) & ~(size_t)7;
     return state + (size_t)Z * 8 + (((size_t)Z * word_bytes + 7) & ~(size_t)7);
+}
+
+// Decision of spin i in the W slices of one parity word, lane m < W = slice
+// 2m+tau, given the exact spatial field h of this lane's slice: the
+// arithmetic, classes and Philox words of the lane-per-slice body.  Returns
+// the flip mask (bit m = slice m flips), wave-uniform.
+template <typename WT, bool PF = false>
+__device__ __forceinline__ WT dense_decide(const ColArgs<WT> &a, double h, WT S_, WT O, int lane, int tau,
+                                           float jp, double Ts, uint32_t g, int i, uint32_t rep)
+{
+    const int W = a.W;
+    const WT one = WT(1);
+    const bool active = lane < W;
+    const int lb = active ? lane : 0;
+    const int ml = tau == 0 ? (lb == 0 ? W - 1 : lb - 1) : lb;
+    const int mr = tau == 0 ? lb : (lb == W - 1 ? 0 : lb + 1);
+    const double si = ((S_ >> lb) & one) ? -1.0 : 1.0;
+    const double ts = (((O >> ml) & one) ? -1.0 : 1.0) + (((O >> mr) & one) ? -1.0 : 1.0);
+    const double ed = -2.0 * si * (h + (double)jp * ts);
+    // ed < -23 T: exp(ed/T) < 1.1e-10 < 2^-33, never accepted (class 0 of
+    // piqmc_fixed_class) -- decided without the division and the exponential
+    const bool pos = ed > 0.0, far = Ts > 0.0 && ed < -23.0 * Ts;
+    bool fl = active && pos;
+    if (__ballot(active && !pos && !far)) {
+        // PF: the exponential and the first two Philox blocks are independent
+        // and computed together, so their latencies overlap (a win where the
+        // decision is the whole chain; where other wavefronts sum the next
+        // field meanwhile, the unused blocks cost more than they save)
+        constexpr int kPer = sizeof(WT) == 4 ? 4 : 2;
+        const uint32_t c3 = uint32_t(tau) << 24;
+        uint32_t o0[4] = {0u, 0u, 0u, 0u}, o1[4] = {0u, 0u, 0u, 0u};
+        if constexpr (PF) {
+            philox4x32_10(g, (uint32_t)i, rep, c3, a.key0, a.key1, o0);
+            philox4x32_10(g, (uint32_t)i, rep, c3 | 1u, a.key0, a.key1, o1);
+        }
+        unsigned int thr = 0;
+        const int cls = (!active || pos || far) ? 0 : piqmc_fixed_class(ed, Ts, &thr);
+        if (cls == 1) fl = true;
+        if (__ballot(cls == 2)) {
+            bool und = cls == 2, lt = false;
+            for (int qd = 0; qd < 32 / kPer && __ballot(und); ++qd) {
+                uint32_t out[4];
+                if (PF && qd == 0) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) out[t] = o0[t];
+                } else if (PF && qd == 1) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) out[t] = o1[t];
+                } else {
+                    philox4x32_10(g, (uint32_t)i, rep, c3 | (uint32_t)qd, a.key0, a.key1, out);
+                }
+#pragma unroll
+                for (int t = 0; t < kPer; ++t) {
+                    const uint32_t ub = (uint32_t)((rword<WT>(out, t) >> lb) & one);
+                    const uint32_t tb = (thr >> (31 - (qd * kPer + t))) & 1u;
+                    if (und && ub != tb) {
+                        lt = ub < tb;
+                        und = false;
+                    }
+                }
+            }
+            if (cls == 2) fl = lt;
+        }
+    }
+    return (WT)__ballot(fl);
 }
 
 template <typename WT, bool SPLIT = false>
@@ -1390,33 +1495,7 @@ __device__ void dense_pipe_replica(const ColArgs<WT> &a, int r, unsigned char *b
                 }
                 const uint32_t g = a.sweep_base + (uint32_t)(pso * a.M + pm);
                 const WT S_ = mine[i], O = other[i];
-                const int ml = ptau == 0 ? (lb == 0 ? W - 1 : lb - 1) : lb;
-                const int mr = ptau == 0 ? lb : (lb == W - 1 ? 0 : lb + 1);
-                const double si = ((S_ >> lb) & one) ? -1.0 : 1.0;
-                const double ts = (((O >> ml) & one) ? -1.0 : 1.0) + (((O >> mr) & one) ? -1.0 : 1.0);
-                const double ed = -2.0 * si * (h + (double)jp * ts);
-                unsigned int thr = 0;
-                const int cls = piqmc_fixed_class(ed, Ts, &thr);
-                bool fl = active && cls == 1;
-                if (__ballot(active && cls == 2)) {
-                    constexpr int kPer = sizeof(WT) == 4 ? 4 : 2;
-                    bool und = active && cls == 2, lt = false;
-                    for (int qd = 0; qd < 32 / kPer && __ballot(und); ++qd) {
-                        uint32_t out[4];
-                        philox4x32_10(g, (uint32_t)i, rep, (uint32_t(ptau) << 24) | (uint32_t)qd, a.key0, a.key1, out);
-#pragma unroll
-                        for (int t = 0; t < kPer; ++t) {
-                            const uint32_t ub = (uint32_t)((rword<WT>(out, t) >> lb) & one);
-                            const uint32_t tb = (thr >> (31 - (qd * kPer + t))) & 1u;
-                            if (und && ub != tb) {
-                                lt = ub < tb;
-                                und = false;
-                            }
-                        }
-                    }
-                    if (active && cls == 2) fl = lt;
-                }
-                lastw = S_ ^ (WT)__ballot(fl);
+                lastw = S_ ^ dense_decide<WT>(a, h, S_, O, lane, ptau, jp, Ts, g, (int)i, rep);
                 if (lane == 0) mine[i] = lastw;
                 if (a.etrace && pq == N - 1 && ptau == 1 && pm == a.M - 1) {
                     double acc = 0.0;
@@ -1566,68 +1645,31 @@ __host__ __device__ inline int fields_pitch(int N) { return N | 1; }
 __host__ __device__ inline size_t dense_fields_bytes(int N, int W, int word_bytes)
 {
     const size_t state = ((size_t)2 * N * word_bytes + 7) & ~(size_t)7;
-    return state + (size_t)W * fields_pitch(N) * sizeof(double);
+    return state + (size_t)W * fields_pitch(N) * sizeof(double) + (((size_t)N * sizeof(int) + 7) & ~(size_t)7);
 }
 
-// Decision of spin i in the W slices of one parity word, lane m < W = slice
-// 2m+tau, given the exact spatial field h of this lane's slice: the
-// arithmetic, classes and Philox words of the lane-per-slice body.  Returns
-// the flip mask (bit m = slice m flips), wave-uniform.
+// Initial local fields of the cached-field sweep, one thread per (replica,
+// parity, slice, spin): the row's sum with a field entry counted as +J, in
+// row order (exact rows give the same double in any order).
 template <typename WT>
-__device__ __forceinline__ WT dense_decide(const ColArgs<WT> &a, double h, WT S_, WT O, int lane, int tau,
-                                           float jp, double Ts, uint32_t g, int i, uint32_t rep)
+__global__ void fields_init_kernel(ColArgs<WT> a)
 {
-    const int W = a.W;
-    const WT one = WT(1);
-    const bool active = lane < W;
-    const int lb = active ? lane : 0;
-    const int ml = tau == 0 ? (lb == 0 ? W - 1 : lb - 1) : lb;
-    const int mr = tau == 0 ? lb : (lb == W - 1 ? 0 : lb + 1);
-    const double si = ((S_ >> lb) & one) ? -1.0 : 1.0;
-    const double ts = (((O >> ml) & one) ? -1.0 : 1.0) + (((O >> mr) & one) ? -1.0 : 1.0);
-    const double ed = -2.0 * si * (h + (double)jp * ts);
-    // ed < -23 T: exp(ed/T) < 1.1e-10 < 2^-33, never accepted (class 0 of
-    // piqmc_fixed_class) -- decided without the division and the exponential
-    const bool pos = ed > 0.0, far = Ts > 0.0 && ed < -23.0 * Ts;
-    bool fl = active && pos;
-    if (__ballot(active && !pos && !far)) {
-        // the exponential and the first two Philox blocks are independent:
-        // computed together, their latencies overlap
-        constexpr int kPer = sizeof(WT) == 4 ? 4 : 2;
-        const uint32_t c3 = uint32_t(tau) << 24;
-        uint32_t o0[4], o1[4];
-        philox4x32_10(g, (uint32_t)i, rep, c3, a.key0, a.key1, o0);
-        philox4x32_10(g, (uint32_t)i, rep, c3 | 1u, a.key0, a.key1, o1);
-        unsigned int thr = 0;
-        const int cls = (!active || pos || far) ? 0 : piqmc_fixed_class(ed, Ts, &thr);
-        if (cls == 1) fl = true;
-        if (__ballot(cls == 2)) {
-            bool und = cls == 2, lt = false;
-            for (int qd = 0; qd < 32 / kPer && __ballot(und); ++qd) {
-                uint32_t out[4];
-                if (qd == 0) {
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) out[t] = o0[t];
-                } else if (qd == 1) {
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) out[t] = o1[t];
-                } else {
-                    philox4x32_10(g, (uint32_t)i, rep, c3 | (uint32_t)qd, a.key0, a.key1, out);
-                }
-#pragma unroll
-                for (int t = 0; t < kPer; ++t) {
-                    const uint32_t ub = (uint32_t)((rword<WT>(out, t) >> lb) & one);
-                    const uint32_t tb = (thr >> (31 - (qd * kPer + t))) & 1u;
-                    if (und && ub != tb) {
-                        lt = ub < tb;
-                        und = false;
-                    }
-                }
-            }
-            if (cls == 2) fl = lt;
+    const int N = a.N, W = a.W, Z = a.Z, NP = fields_pitch(N);
+    const size_t tot = (size_t)a.R * 2 * W * N;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
+        const int j = (int)(t % N);
+        const size_t rest = t / N;
+        const int m = (int)(rest % W), tau = (int)((rest / W) % 2), r = (int)(rest / (2 * (size_t)W));
+        const WT *w = a.packed + ((size_t)r * 2 + tau) * N;
+        double h = 0.0;
+        for (int e = 0; e < Z; ++e) {
+            const int2 rw = a.rows[(size_t)j * Z + e];
+            const double J = (double)__int_as_float(rw.y);
+            const WT wj = rw.x == j ? WT(0) : w[rw.x];
+            h += ((wj >> m) & WT(1)) ? -J : J;
         }
+        a.fields[(((size_t)r * 2 + tau) * W + m) * NP + j] = h;
     }
-    return (WT)__ballot(fl);
 }
 
 template <typename WT>
@@ -1637,29 +1679,18 @@ __device__ void dense_fields_replica(const ColArgs<WT> &a, int r, unsigned char 
     WT *st = reinterpret_cast<WT *>(base);
     double *tab = reinterpret_cast<double *>(base + (((size_t)2 * N * sizeof(WT) + 7) & ~(size_t)7));
     const WT one = WT(1);
-    const bool active = lane < W;
-    const int lb = active ? lane : 0;
+    const int lb = lane < W ? lane : 0;
     const uint32_t rep = a.replica_base + (uint32_t)r;
     WT *gst = a.packed + (size_t)r * 2 * N;
     double *gt = a.fields + (size_t)r * 2 * W * NP;  // [2][W][NP]
+    // the spin order, then the state, into LDS (the fields of both parities
+    // were computed by fields_init_kernel)
+    int *ord = reinterpret_cast<int *>(tab + (size_t)W * NP);
+    for (int q = lane; q < N; q += 64) ord[q] = a.order[q];
     for (int q = lane; q < 2 * N; q += 64) st[q] = gst[q];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // fields of both parities from the rows (a field entry counts with +J)
-    for (int tau = 0; tau < 2; ++tau) {
-        const WT *w = st + tau * N;
-        for (int j = 0; j < N; ++j) {
-            double h = 0.0;
-            for (int e = 0; e < Z; ++e) {
-                const int2 rw = a.rows[(size_t)j * Z + e];
-                const double J = (double)__int_as_float(rw.y);
-                const WT wj = rw.x == j ? WT(0) : w[rw.x];
-                h += ((wj >> lb) & one) ? -J : J;
-            }
-            if (active) gt[((size_t)tau * W + lane) * NP + j] = h;
-        }
-    }
     constexpr int kPre = 4;  // row entries per lane held in registers (rows up to 256)
     int2 pre[kPre], cur[kPre];
     auto fetch_row = [&](int i) {
@@ -1677,18 +1708,18 @@ __device__ void dense_fields_replica(const ColArgs<WT> &a, int r, unsigned char 
                 const WT *other = st + (1 - tau) * N;
                 double *gtt = gt + (size_t)tau * W * NP;
                 for (int t = lane; t < W * NP; t += 64) tab[t] = gtt[t];
-                fetch_row(__builtin_amdgcn_readfirstlane(a.order[0]));
+                fetch_row(__builtin_amdgcn_readfirstlane(ord[0]));
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 for (int q = 0; q < N; ++q) {
-                    const int i = __builtin_amdgcn_readfirstlane(a.order[q]);
+                    const int i = __builtin_amdgcn_readfirstlane(ord[q]);
 #pragma unroll
                     for (int k = 0; k < kPre; ++k) cur[k] = pre[k];
-                    if (q + 1 < N) fetch_row(__builtin_amdgcn_readfirstlane(a.order[q + 1]));
+                    if (q + 1 < N) fetch_row(__builtin_amdgcn_readfirstlane(ord[q + 1]));
                     const double h = tab[(size_t)lb * NP + i];
                     const WT S_ = mine[i], O = other[i];
-                    const WT bal = dense_decide<WT>(a, h, S_, O, lane, tau, jp, Ts, g, i, rep);
+                    const WT bal = dense_decide<WT, true>(a, h, S_, O, lane, tau, jp, Ts, g, i, rep);
                     if (lane == 0) mine[i] = S_ ^ bal;
                     // neighbours' fields in the slices where spin i flipped:
                     // + 2 J s_i^new (old bit set = -1, so the new spin is +1).
@@ -1756,7 +1787,7 @@ __device__ void dense_fields_replica(const ColArgs<WT> &a, int r, unsigned char 
 // One replica per wavefront; as many wavefronts per workgroup as the launch
 // gives (the host keeps the LDS of all of them within 160 KiB).
 template <typename WT>
-__global__ void dense_fields_kernel(ColArgs<WT> a)
+__global__ void __launch_bounds__(256) dense_fields_kernel(ColArgs<WT> a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int w = threadIdx.x >> 6, r = blockIdx.x * (blockDim.x >> 6) + w;
@@ -1969,7 +2000,7 @@ size_t colored_scratch_bytes(const piqmc_problem *pb, int R, int P, int S)
 {
     const size_t wb = P / 2 <= 32 ? 4 : 8;
     size_t b = CallBuffers::round_up(sizeof(float) * (S ? 2 * S : 1));
-    if (use_dense(pb) && pb->dense_variant == 6)
+    if (use_dense(pb) && (pb->dense_variant == 6 || (pb->dense_variant == 0 && pb->exact_rows && pb->fields_ok)))
         b += CallBuffers::round_up(sizeof(double) * 2 * (size_t)(P / 2) * (pb->N | 1) * R);
     if (!use_dense(pb)) {
         if (pb->pmj && S)
@@ -2005,10 +2036,15 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         bool wg = false, pipe = false, split = false;
         int pw = 4;
         size_t lwg = 0;
+        bool fields_auto = false;
         if (pb->dense_variant == 0) {
+            // cached fields while replicas are few (256-spin SK, P=128: 33 ms
+            // against 46 ms pipelined at 4 replicas, 20 against 18 at 64)
+            fields_auto = pb->exact_rows && pb->fields_ok && R <= 32 && dense_wg_enabled() &&
+                          dense_fields_bytes(N, W, sizeof(WT)) <= 160 * 1024;
             // a workgroup per replica while replicas are few (1.2-1.9x up to 1024
             // 256-spin replicas); from 4096 on, a wavefront each is 1.3x faster
-            const bool wg0 = pb->exact_rows && dense_wg_enabled() && R < 2048;
+            const bool wg0 = !fields_auto && pb->exact_rows && dense_wg_enabled() && R < 2048;
             pw = dense_pipe_waves(pb->Z, W, R);
             pipe = wg0 && dense_pipe_enabled() && dense_pipe_bytes(N, pb->Z, sizeof(WT), pw - 1) <= 160 * 1024;
             lwg = pipe ? dense_pipe_bytes(N, pb->Z, sizeof(WT), pw - 1) : dense_wg_bytes(N, pb->Z, sizeof(WT));
@@ -2033,7 +2069,7 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
                                                       "summable rows and a row that fits the LDS");
             wg = true;
         }
-        const bool fields = pb->dense_variant == 6;
+        const bool fields = pb->dense_variant == 6 || fields_auto;
         if (!wg && !fields && lds > 160 * 1024) return piqmc::set_error(PIQMC_EINVAL, "dense sweep: state exceeds LDS");
         float *d_jp = nullptr;  // [S] J_perp, then [S] temperatures
         WT *d_packed = nullptr;
@@ -2056,9 +2092,10 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         if (fields) {
             // one replica per wavefront, as many per workgroup as block and LDS allow
             const size_t per = dense_fields_bytes(N, W, sizeof(WT));
-            const int wf = std::max(1, std::min(wpb, (int)((160 * 1024) / per)));
+            const int wf = std::max(1, std::min(std::min(wpb, 4), (int)((160 * 1024) / per)));
             (void)hipFuncSetAttribute((const void *)dense_fields_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)(per * wf));
+            fields_init_kernel<WT><<<grid_for((size_t)R * 2 * W * N, 256), 256, 0, st>>>(a);
             dense_fields_kernel<WT><<<(R + wf - 1) / wf, 64 * wf, per * wf, st>>>(a);
         } else if (wg && pipe) {
             // one replica per workgroup, one spin decided while the next is summed
