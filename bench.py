@@ -247,8 +247,11 @@ def c5_leg(steps=2000, replicas=1024):
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
 
 
-def gemm_leg(shapes=((1024, 1024, 1728), (1024, 1024, 1024), (1024, 1728, 1024),
-                     (2048, 512, 256), (2048, 512, 512), (2048, 256, 512)), reps=20):
+GEMM_SHAPES = ((1024, 1024, 1728), (1024, 1024, 1024), (1024, 1728, 1024),
+               (2048, 512, 256), (2048, 512, 512), (2048, 256, 512))
+
+
+def gemm_leg(shapes=GEMM_SHAPES, reps=20):
     """The network's BF16 MFMA GEMM (C = A B^T, FP32 accumulation) at the
     shapes of the two MADE workloads (batch x out x in), timed with HIP events
     on the stream the kernel runs on."""
@@ -276,6 +279,44 @@ def gemm_leg(shapes=((1024, 1024, 1728), (1024, 1024, 1024), (1024, 1728, 1024),
     return out
 
 
+def gemm_tiles_leg(shapes=GEMM_SHAPES + ((4096, 1024, 1728), (8192, 1024, 1024)), reps=20):
+    """Every tile of the network GEMM (forced through PIQMC_GEMM_TILE: 1
+    64x64, 2 128x64, 3 128x128, 4 and 5 64x64 with K over two or four
+    wavefront groups, 0 the automatic choice) at the gemm_leg shapes, and
+    torch.matmul (hipBLASLt) on the same shapes as the library reference
+    point.  Event times include the host's launch rate; a kernel trace of
+    this mode gives the kernel durations."""
+    import torch
+    rows = []
+    saved = os.environ.get("PIQMC_GEMM_TILE")
+    try:
+        for tile in "123450":
+            os.environ["PIQMC_GEMM_TILE"] = tile
+            for g in gemm_leg(shapes=shapes, reps=reps):
+                rows.append(dict(g, tile=int(tile)))
+    finally:
+        if saved is None:
+            os.environ.pop("PIQMC_GEMM_TILE", None)
+        else:
+            os.environ["PIQMC_GEMM_TILE"] = saved
+    st = torch.cuda.current_stream()
+    for M, N, K in shapes:
+        A = torch.randn(M, K, device="cuda").to(torch.bfloat16)
+        B = torch.randn(N, K, device="cuda").to(torch.bfloat16)
+        torch.matmul(A, B.t())
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            torch.matmul(A, B.t())
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        tf = 2.0 * M * N * K / (ms * 1e-3) / 1e12
+        rows.append({"M": M, "N": N, "K": K, "us": ms * 1e3, "tflops": tf,
+                     "mfma_frac": tf / MFMA_BF16_PEAK_TFLOPS, "tile": "torch.matmul"})
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,6 +327,8 @@ def main():
     ap.add_argument("--rpw", type=int, default=0, help="replicas per workgroup (0: autotune)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one anneal plus a 1 GiB device copy (for rocprofv3 --pmc runs)")
+    ap.add_argument("--gemm-tiles", action="store_true",
+                    help="time every tile of the network GEMM and torch.matmul, then exit")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-replay", action="store_true")
     ap.add_argument("--replay-steps", type=int, default=500)
@@ -304,6 +347,11 @@ def main():
     import numpy as np
     import torch
     from piqmc import device, instances
+
+    if args.gemm_tiles:
+        for row in gemm_tiles_leg():
+            print(json.dumps(row), flush=True)
+        return
 
     dev = torch.device("cuda:0")
     L, P, T, S, R = 32, 64, 1.0 / 64, 500, args.replicas

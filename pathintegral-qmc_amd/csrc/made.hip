@@ -22,6 +22,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 
 #include "piqmc_internal.h"
@@ -230,14 +231,275 @@ __global__ __launch_bounds__(256) void gemm_nt_bf16_pipe(const uint16_t *A, int 
         }
 }
 
+// Larger tiles: BM x BN per workgroup of four wavefronts (2 x 2), each
+// wavefront (BM/2) x (BN/2) as (BM/32) x (BN/32) blocks of 16 x 16 MFMAs, so
+// every fragment read from LDS feeds BN/32 or BM/32 MFMAs instead of two.
+// K-stages of 64 through a double-buffered LDS tile; the next stage's global
+// loads are in flight in registers while the current one's MFMAs run.  Rows
+// past M / N read a valid row and are zeroed on the way to LDS.
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void gemm_nt_bf16_tile(const uint16_t *A, int lda, const uint16_t *B, int ldb,
+                                                         int M, int N, int K, const float *bias, int relu,
+                                                         float *Cf, uint16_t *Cb, int ldc)
+{
+    constexpr int TM = BM / 32, TN = BN / 32;          // 16 x 16 blocks per wavefront
+    constexpr int CA = BM * 8 / 256, CB = BN * 8 / 256;  // 16-byte chunks per thread per stage
+    __shared__ __align__(16) uint16_t As[2][BM][PKP];
+    __shared__ __align__(16) uint16_t Bs[2][BN][PKP];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int wm = (wid >> 1) * (BM / 2), wn = (wid & 1) * (BN / 2);
+    f32x4 acc[TM][TN];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const uint16_t *pa[CA], *pb[CB];
+    uint32_t ma[CA], mb[CB];
+    int ra[CA], ca[CA], rb[CB], cb[CB];
+#pragma unroll
+    for (int u = 0; u < CA; ++u) {
+        const int c = tid + 256 * u;
+        ra[u] = c >> 3;
+        ca[u] = (c & 7) * 8;
+        const bool in = m0 + ra[u] < M;
+        pa[u] = A + (size_t)(in ? m0 + ra[u] : 0) * lda + ca[u];
+        ma[u] = in ? ~0u : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < CB; ++u) {
+        const int c = tid + 256 * u;
+        rb[u] = c >> 3;
+        cb[u] = (c & 7) * 8;
+        const bool in = n0 + rb[u] < N;
+        pb[u] = B + (size_t)(in ? n0 + rb[u] : 0) * ldb + cb[u];
+        mb[u] = in ? ~0u : 0u;
+    }
+    auto masked = [](uint4 v, uint32_t m) { return make_uint4(v.x & m, v.y & m, v.z & m, v.w & m); };
+    uint4 va[CA], vb[CB];
+    auto fetch = [&](int stage) __attribute__((always_inline)) {
+        const int k0 = stage * PK;
+#pragma unroll
+        for (int u = 0; u < CA; ++u) va[u] = *reinterpret_cast<const uint4 *>(pa[u] + k0);
+#pragma unroll
+        for (int u = 0; u < CB; ++u) vb[u] = *reinterpret_cast<const uint4 *>(pb[u] + k0);
+    };
+    auto stash = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int u = 0; u < CA; ++u) *reinterpret_cast<uint4 *>(&As[buf][ra[u]][ca[u]]) = masked(va[u], ma[u]);
+#pragma unroll
+        for (int u = 0; u < CB; ++u) *reinterpret_cast<uint4 *>(&Bs[buf][rb[u]][cb[u]]) = masked(vb[u], mb[u]);
+    };
+    const int ns = K / PK;
+    fetch(0);
+    stash(0);
+    if (ns > 1) fetch(1);
+    __syncthreads();
+    for (int s = 0; s < ns; ++s) {
+        const int buf = s & 1;
+#pragma unroll
+        for (int kk = 0; kk < PK; kk += 32) {
+            bf16x8 af[TM], bfr[TN];
+#pragma unroll
+            for (int t = 0; t < TM; ++t)
+                af[t] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4 *>(&As[buf][wm + t * 16 + (lane & 15)][kk + (lane >> 4) * 8]));
+#pragma unroll
+            for (int t = 0; t < TN; ++t)
+                bfr[t] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4 *>(&Bs[buf][wn + t * 16 + (lane & 15)][kk + (lane >> 4) * 8]));
+#pragma unroll
+            for (int a = 0; a < TM; ++a)
+#pragma unroll
+                for (int b = 0; b < TN; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        }
+        if (s + 1 < ns) {
+            stash(buf ^ 1);                 // stage s + 1, loaded during stage s
+            if (s + 2 < ns) fetch(s + 2);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+            const int col = n0 + wn + b * 16 + (lane & 15);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = m0 + wm + a * 16 + (lane >> 4) * 4 + j;
+                if (row < M && col < N) {
+                    float v = acc[a][b][j];
+                    if (bias) v += bias[col];
+                    if (relu) v = v > 0.f ? v : 0.f;
+                    if (Cf) Cf[(size_t)row * ldc + col] = v;
+                    if (Cb) Cb[(size_t)row * ldc + col] = f2bf_bits(v);
+                }
+            }
+        }
+}
+
+// K split inside the workgroup: KG groups of four wavefronts share one 64 x 64
+// output tile, group g taking the K-stages s = g, g + KG, g + 2 KG, ...  Each
+// group has its own double-buffered stage in LDS and a two-stage register
+// ring of global loads, so a SIMD holds KG wavefronts of the same tile and
+// one's LDS reads and barriers hide behind the others' MFMAs.  At the end the
+// groups' partial tiles meet in LDS and are summed in group order (the result
+// does not depend on timing).  Loads past the last stage repeat it, so every
+// group issues the same loads; stages past the end are not computed.
+template <int KG>
+__global__ __launch_bounds__(256 * KG) void gemm_nt_bf16_splitk(const uint16_t *A, int lda, const uint16_t *B,
+                                                              int ldb, int M, int N, int K, const float *bias,
+                                                              int relu, float *Cf, uint16_t *Cb, int ldc)
+{
+    constexpr int kBuf = GT * PKP;                        // bf16 per stage buffer of A or of B
+    constexpr int kRedPitch = GT + 4;                     // floats per row of a partial tile
+    constexpr size_t kStageBytes = (size_t)KG * 4 * kBuf * 2;  // per group: A x 2 buffers, B x 2 buffers
+    constexpr size_t kRedBytes = (size_t)KG * GT * kRedPitch * 4;
+    __shared__ __align__(16) unsigned char smem[kStageBytes > kRedBytes ? kStageBytes : kRedBytes];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int grp = wid >> 2, gt = tid & 255, gw = wid & 3;
+    uint16_t *As = reinterpret_cast<uint16_t *>(smem) + (size_t)grp * 4 * kBuf;  // [buf][row][k]
+    uint16_t *Bs = As + 2 * kBuf;
+    auto a_at = [&](int buf, int r, int k) -> uint16_t * { return As + buf * kBuf + r * PKP + k; };
+    auto b_at = [&](int buf, int r, int k) -> uint16_t * { return Bs + buf * kBuf + r * PKP + k; };
+    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    const int wm = (gw >> 1) * 32, wn = (gw & 1) * 32;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int sr = gt >> 3, sc = (gt & 7) * 8;
+    const bool a0 = m0 + sr < M, a1 = m0 + sr + 32 < M, b0 = n0 + sr < N, b1 = n0 + sr + 32 < N;
+    const uint16_t *pa0 = A + (size_t)(a0 ? m0 + sr : 0) * lda + sc;
+    const uint16_t *pa1 = A + (size_t)(a1 ? m0 + sr + 32 : 0) * lda + sc;
+    const uint16_t *pb0 = B + (size_t)(b0 ? n0 + sr : 0) * ldb + sc;
+    const uint16_t *pb1 = B + (size_t)(b1 ? n0 + sr + 32 : 0) * ldb + sc;
+    const uint32_t ma0 = a0 ? ~0u : 0u, ma1 = a1 ? ~0u : 0u, mb0 = b0 ? ~0u : 0u, mb1 = b1 ? ~0u : 0u;
+    auto masked = [](uint4 v, uint32_t m) { return make_uint4(v.x & m, v.y & m, v.z & m, v.w & m); };
+    const int ns = K / PK, rounds = (ns + KG - 1) / KG;
+    uint4 ra0[2], ra1[2], rb0[2], rb1[2];
+    auto fetch = [&](int slot, int round) __attribute__((always_inline)) {
+        const int k0 = min(round * KG + grp, ns - 1) * PK;
+        ra0[slot] = *reinterpret_cast<const uint4 *>(pa0 + k0);
+        ra1[slot] = *reinterpret_cast<const uint4 *>(pa1 + k0);
+        rb0[slot] = *reinterpret_cast<const uint4 *>(pb0 + k0);
+        rb1[slot] = *reinterpret_cast<const uint4 *>(pb1 + k0);
+    };
+    auto stash = [&](int slot, int buf) __attribute__((always_inline)) {
+        *reinterpret_cast<uint4 *>(a_at(buf, sr, sc)) = masked(ra0[slot], ma0);
+        *reinterpret_cast<uint4 *>(a_at(buf, sr + 32, sc)) = masked(ra1[slot], ma1);
+        *reinterpret_cast<uint4 *>(b_at(buf, sr, sc)) = masked(rb0[slot], mb0);
+        *reinterpret_cast<uint4 *>(b_at(buf, sr + 32, sc)) = masked(rb1[slot], mb1);
+    };
+    fetch(0, 0);
+    fetch(1, 1);
+    stash(0, 0);
+    __syncthreads();
+    // round r uses register slot r & 1 and LDS buffer r & 1 (slots indexed
+    // by constants, so the ring stays in registers)
+    auto step = [&](int r, const int slot) __attribute__((always_inline)) {
+        const int buf = slot;
+        fetch(slot, r + 2);                               // slot held round r, already in LDS
+        if (r * KG + grp < ns) {
+#pragma unroll
+            for (int kk = 0; kk < PK; kk += 32) {
+                bf16x8 af[2], bfr[2];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    af[t] = __builtin_bit_cast(
+                        bf16x8, *reinterpret_cast<const uint4 *>(a_at(buf, wm + t * 16 + (lane & 15), kk + (lane >> 4) * 8)));
+                    bfr[t] = __builtin_bit_cast(
+                        bf16x8, *reinterpret_cast<const uint4 *>(b_at(buf, wn + t * 16 + (lane & 15), kk + (lane >> 4) * 8)));
+                }
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+            }
+        }
+        stash(slot ^ 1, buf ^ 1);                         // round r + 1 (unused after the last)
+        __syncthreads();
+    };
+    for (int r = 0; r < rounds; r += 2) {
+        step(r, 0);
+        if (r + 1 < rounds) step(r + 1, 1);
+    }
+    float *red = reinterpret_cast<float *>(smem);         // [KG][GT][kRedPitch]; stages are done
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                red[((size_t)grp * GT + wm + a * 16 + (lane >> 4) * 4 + j) * kRedPitch + wn + b * 16 + (lane & 15)] =
+                    acc[a][b][j];
+    __syncthreads();
+    for (int idx = tid; idx < GT * GT; idx += 256 * KG) {
+        const int rl = idx / GT, cl = idx % GT, row = m0 + rl, col = n0 + cl;
+        if (row < M && col < N) {
+            float v = 0.f;
+#pragma unroll
+            for (int g = 0; g < KG; ++g) v += red[((size_t)g * GT + rl) * kRedPitch + cl];
+            if (bias) v += bias[col];
+            if (relu) v = v > 0.f ? v : 0.f;
+            if (Cf) Cf[(size_t)row * ldc + col] = v;
+            if (Cb) Cb[(size_t)row * ldc + col] = f2bf_bits(v);
+        }
+    }
+}
+
+// Tile of the pipelined GEMM: 0 automatic, 1 64x64, 2 128x64, 3 128x128,
+// 4 64x64 with K in two groups, 5 64x64 with K in four groups
+// (PIQMC_GEMM_TILE forces one, for comparisons).
+int gemm_tile_choice(int M, int N, int K)
+{
+    const char *e = std::getenv("PIQMC_GEMM_TILE");
+    if (e && e[0] >= '1' && e[0] <= '5') return e[0] - '0';
+    // Measured on MI355X: the K split wins while 64 x 64 tiles leave CUs
+    // idle; the wide tiles win once they alone give each CU two of them.
+    const long long t64 = (long long)((M + 63) / 64) * ((N + 63) / 64);
+    const long long t128 = (long long)((M + 127) / 128) * ((N + 127) / 128);
+    const long long t12864 = (long long)((M + 127) / 128) * ((N + 63) / 64);
+    const int ns = K / PK;
+    if (t64 <= 256 && ns >= 8) return 5;
+    if (t64 <= 512 && ns >= 4) return 4;
+    if (t128 >= 512) return 3;
+    if (t12864 >= 512) return 2;
+    return 1;
+}
+
 void gemm(const uint16_t *A, int lda, const uint16_t *B, int ldb, int M, int N, int K, const float *bias,
           int relu, float *Cf, uint16_t *Cb, int ldc, hipStream_t st)
 {
-    dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT);
-    if (K % PK == 0 && lda % 8 == 0 && ldb % 8 == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0)
+    if (K % PK == 0 && lda % 8 == 0 && ldb % 8 == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0) {
+        const int tile = gemm_tile_choice(M, N, K);
+        if (tile == 4 || tile == 5) {
+            dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT);
+            if (tile == 5)
+                gemm_nt_bf16_splitk<4><<<grid, 1024, 0, st>>>(A, lda, B, ldb, M, N, K, bias, relu, Cf, Cb, ldc);
+            else
+                gemm_nt_bf16_splitk<2><<<grid, 512, 0, st>>>(A, lda, B, ldb, M, N, K, bias, relu, Cf, Cb, ldc);
+            return;
+        }
+        if (tile == 3) {
+            dim3 grid((N + 127) / 128, (M + 127) / 128);
+            gemm_nt_bf16_tile<128, 128><<<grid, 256, 0, st>>>(A, lda, B, ldb, M, N, K, bias, relu, Cf, Cb, ldc);
+            return;
+        }
+        if (tile == 2) {
+            dim3 grid((N + 63) / 64, (M + 127) / 128);
+            gemm_nt_bf16_tile<128, 64><<<grid, 256, 0, st>>>(A, lda, B, ldb, M, N, K, bias, relu, Cf, Cb, ldc);
+            return;
+        }
+        dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT);
         gemm_nt_bf16_pipe<<<grid, 256, 0, st>>>(A, lda, B, ldb, M, N, K, bias, relu, Cf, Cb, ldc);
-    else
-        gemm_nt_bf16<<<grid, 256, 0, st>>>(A, lda, B, ldb, M, N, K, bias, relu, Cf, Cb, ldc);
+        return;
+    }
+    dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT);
+    gemm_nt_bf16<<<grid, 256, 0, st>>>(A, lda, B, ldb, M, N, K, bias, relu, Cf, Cb, ldc);
 }
 
 // ------------------------------------------------------------------ elementwise

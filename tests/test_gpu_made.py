@@ -35,6 +35,27 @@ def test_gemm_bf16_matches_fp32_reference(gpu, M, N, K):
     torch.testing.assert_close(C.cpu(), ref, rtol=1e-4, atol=1e-3)
 
 
+@pytest.mark.parametrize("tile", ["1", "2", "3", "4", "5"])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 64), (130, 70, 192), (1000, 300, 832), (1024, 1024, 1728),
+                                   (2048, 512, 320), (257, 129, 64)])
+def test_gemm_tiles_match_fp32_reference(gpu, monkeypatch, tile, M, N, K):
+    """Every tile of the pipelined GEMM (64x64, 128x64, 128x128, 64x64 with K
+    split over two or four wavefront groups), forced, including ragged edges
+    in M and N and fewer K-stages than groups."""
+    import torch
+    from piqmc import _lib
+    monkeypatch.setenv("PIQMC_GEMM_TILE", tile)
+    g = torch.Generator(device="cpu").manual_seed(7 * M + N + K)
+    A = torch.randn(M, K, generator=g).to(torch.bfloat16)
+    B = torch.randn(N, K, generator=g).to(torch.bfloat16)
+    ref = A.float() @ B.float().T
+    Ad, Bd = A.view(torch.int16).to(gpu), B.view(torch.int16).to(gpu)
+    C = torch.full((M, N), float("nan"), dtype=torch.float32, device=gpu)
+    _lib.call("piqmc_gemm_bf16_nt", Ad.data_ptr(), Bd.data_ptr(), M, N, K, C.data_ptr(), None)
+    torch.cuda.synchronize()
+    torch.testing.assert_close(C.cpu(), ref, rtol=1e-4, atol=1e-3)
+
+
 def _made(N=24, H1=40, H2=36, seed=3):
     from piqmc.made import MADE
     return MADE(N, H1, H2, seed=seed)
