@@ -239,6 +239,22 @@ __device__ __forceinline__ void field_classes(double h, int z, float jp, double 
 // CACHED (with ZU > 0): two colors of at most two spins per thread each; every
 // thread keeps its spins' indices and neighbour lists in registers for the
 // whole schedule instead of reloading them in every phase.
+// Rotation of a parity word's W slices left by tsa (right by tsb = W - tsa):
+// the Trotter neighbours of the other parity.  32-bit words take one funnel
+// shift: (O : O << rls) >> rsh keeps (O << tsa) | (O >> (W - tsa)) in its
+// low W bits (rls = 32 - W, rsh = 32 - tsa; W = 1 with tsa = 0 is O itself,
+// rls = rsh = 0).  Bits past W are masked by the caller.
+template <typename WT>
+__device__ inline WT rot_slices(WT O, int tsa, int tsb, int, int)
+{
+    return (O << tsa) | (O >> tsb);
+}
+template <>
+__device__ inline uint32_t rot_slices<uint32_t>(uint32_t O, int, int, int rls, int rsh)
+{
+    return __builtin_amdgcn_alignbit(O, O << rls, (uint32_t)rsh);
+}
+
 template <typename WT, bool PMJ, bool UZ = false, bool FLD = false, int ZU = 0, bool CACHED = false, bool CQ = false>
 __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned char *smem)
 {
@@ -273,9 +289,16 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                 cI[c][k] = a.order[qs];
                 cF[c][k] = a.cinfo[qs];
 #pragma unroll
-                for (int z = 0; z < ZC; ++z) cN[c][k][z] = a.cnb[(size_t)z * N + qs];
+                for (int z = 0; z < ZC; ++z) cN[c][k][z] = a.cnb[(size_t)z * N + qs] * (int)sizeof(WT);  // bytes
             }
     }
+
+    // a neighbour's word: cached neighbours are byte offsets (no shift per word)
+    auto nbword = [](const WT *base, int nb) __attribute__((always_inline)) -> WT {
+        if constexpr (CACHED) return *reinterpret_cast<const WT *>(reinterpret_cast<const char *>(base) + nb);
+        else return base[nb];
+    };
+    constexpr int KMAX = 8;  // thresholds clamp at 8 ("count >= 8" is empty)
 
     for (int s = 0; s < a.S; ++s) {
         __syncthreads();
@@ -287,9 +310,9 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
         // uniform-degree lattices: the class thresholds of this step are wave-uniform
         const uint32_t zt_u = (PMJ && UZ) ? __builtin_amdgcn_readfirstlane(a.ztab[s * 8 + a.zuni]) : 0u;
         // its six thresholds (accept / stochastic per Trotter class), clamped to the table
-        const int ka0 = min((int)(zt_u & 15), 8), ks0 = min((int)((zt_u >> 4) & 15), 8);
-        const int ka1 = min((int)((zt_u >> 8) & 15), 8), ks1 = min((int)((zt_u >> 12) & 15), 8);
-        const int ka2 = min((int)((zt_u >> 16) & 15), 8), ks2 = min((int)((zt_u >> 20) & 15), 8);
+        const int ka0 = min((int)(zt_u & 15), KMAX), ks0 = min((int)((zt_u >> 4) & 15), KMAX);
+        const int ka1 = min((int)((zt_u >> 8) & 15), KMAX), ks1 = min((int)((zt_u >> 12) & 15), KMAX);
+        const int ka2 = min((int)((zt_u >> 16) & 15), KMAX), ks2 = min((int)((zt_u >> 20) & 15), KMAX);
         // and its stochastic (u_t, u_s) classes with their draw thresholds (the
         // first three; more take the table walk)
         int un = 0, ut_s[3] = {0, 0, 0}, us_s[3] = {0, 0, 0};
@@ -325,6 +348,8 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                 const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
                 for (int tau = 0; tau < 2; ++tau) {
                     const int tsa = tau == 0 ? 1 : W - 1, tsb = tau == 0 ? W - 1 : 1;  // rotation by one slice
+                    // the same rotation as one funnel shift of (O : O << rls) right by rsh
+                    const int rls = (W == 1 && tau == 1) ? 0 : 32 - W, rsh = (32 - tsa) & 31;
                     // The top cnt queued words, one per lane: each word's stochastic
                     // bits meet the same bit-sliced uniforms, thresholds and early
                     // exit as in the inline path, and the accepted ones are XORed
@@ -456,7 +481,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 // Trotter neighbours of slice 2m+tau: bits m of O and of O
                                 // rotated by one slice (left for tau 0, right for tau 1);
                                 // everything below is symmetric in the two
-                                const WT Lw = O, Rw = ((O << tsa) | (O >> tsb)) & valid;
+                                const WT Lw = O, Rw = rot_slices<WT>(O, tsa, tsb, rls, rsh) & valid;
                                 const WT uL = S_ ^ Lw, uR = S_ ^ Rw;  // 1 = antiparallel Trotter bond
                                 WT flips = 0;
                                 if (PMJ) {
@@ -468,7 +493,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                         WT x[ZU];
     #pragma unroll
                                         for (int z = 0; z < ZU; ++z)
-                                            x[z] = xor3<WT>(S_, mine[nb[z]], ((info >> z) & 1u) ? ~WT(0) : WT(0));
+                                            x[z] = xor3<WT>(S_, nbword(mine, nb[z]), ((info >> z) & 1u) ? ~WT(0) : WT(0));
                                         if constexpr (ZU == 4) {
                                             const WT s1 = xor3<WT>(x[0], x[1], x[2]), k1 = maj3<WT>(x[0], x[1], x[2]);
                                             c0 = s1 ^ x[3];
@@ -487,7 +512,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
     #pragma unroll
                                     for (int z = 0; z < 8; ++z) {
                                         if (ZU > 0 ? z < ZU : (z < a.zmax && z < zi)) {
-                                            const WT x = S_ ^ mine[nb[z]] ^ (((info >> z) & 1u) ? ~WT(0) : WT(0));
+                                            const WT x = S_ ^ nbword(mine, nb[z]) ^ (((info >> z) & 1u) ? ~WT(0) : WT(0));
                                             const WT k0 = c0 & x;
                                             c0 ^= x;
                                             const WT k1 = c1 & k0;
@@ -499,7 +524,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                     // Trotter classes; bits past W are zero in every class
                                     const WT e0 = ~(uL | uR) & valid, e1 = uL ^ uR, e2 = uL & uR;
                                     WT X;
-                                    WT ge[9];  // "count >= k" masks (uniform-degree path)
+                                    WT ge[KMAX + 1];  // "count >= k" masks (uniform-degree path)
                                     if (FLD) {
                                         // per-spin thresholds for (sigma, u_t): sigma = +1 bits are ~S_
                                         flips = 0;
