@@ -423,6 +423,52 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                         __builtin_amdgcn_wave_barrier();
                         if (q2 >= 64) drain2(64);
                     };
+                    // Single class, end of a color phase with at most 64 records
+                    // left: the fresh ones (Philox block 0 onwards) and the
+                    // undecided ones (block 1 onwards) share one pass, each lane
+                    // starting at its own block, so the phase ends with one
+                    // partial batch instead of two.
+                    auto drain_last = [&]() __attribute__((always_inline)) {
+                        __builtin_amdgcn_wave_barrier();
+                        const int lane = tid & 63;
+                        if (lane < qn + q2) {
+                            int i, rr, qd0;
+                            WT m0, lt, und;
+                            if (lane < qn) {
+                                const uint2 r2 = wq1[lane];
+                                i = (int)(r2.x & 0xffffu);
+                                rr = (int)(r2.x >> 16);
+                                m0 = (WT)r2.y;
+                                lt = 0;
+                                und = m0;
+                                qd0 = 0;
+                            } else {
+                                const uint4 rec = wq2[lane - qn];
+                                i = (int)(rec.x & 0xffffu);
+                                rr = (int)(rec.x >> 16);
+                                m0 = (WT)rec.y;
+                                lt = (WT)rec.z;
+                                und = (WT)rec.w;
+                                qd0 = 1;
+                            }
+                            const uint32_t rep = a.replica_base + (uint32_t)(r0 + rr);
+                            for (int qd = qd0; qd < 8 && und; ++qd) {
+                                uint32_t out[4];
+                                philox4x32_10(g, (uint32_t)i, rep, (uint32_t(tau) << 24) | (uint32_t)qd, a.key0, a.key1, out);
+    #pragma unroll
+                                for (int t = 0; t < 4; ++t) {
+                                    const WT Rb = rword<WT>(out, t);
+                                    const WT tb = ((th_s[0] >> (31 - (qd * 4 + t))) & 1u) ? m0 : WT(0);
+                                    lt |= und & ~Rb & tb;
+                                    und &= ~(Rb ^ tb);
+                                }
+                            }
+                            st[((size_t)rr * 2 + tau) * N + i] ^= lt;
+                        }
+                        qn = 0;
+                        q2 = 0;
+                        __builtin_amdgcn_wave_barrier();
+                    };
                     // NS: classes of the step (1, or up to 3), a compile-time constant.
                     auto drain = [&](const int cnt, auto NS) __attribute__((always_inline)) {
                         __builtin_amdgcn_wave_barrier();
@@ -772,8 +818,12 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 spin(q >= 0, q, i, info, ZU, nb);
                             }
                             if constexpr (CQ && MODE == 1) {
-                                if (qn > 0) drain1(qn);
-                                while (q2 > 0) drain2(min(q2, 64));
+                                // full batches first, then one mixed pass over the rest
+                                while (qn + q2 > 64) {
+                                    if (qn > 0) drain1(min(qn, 64));
+                                    else drain2(64);
+                                }
+                                if (qn + q2 > 0) drain_last();
                             } else if (CQ && !DET && qn > 0) {
                                 drain(qn, std::integral_constant<int, 3>());
                             }
@@ -806,8 +856,12 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 spin(act, q, i, info, zi, nb);
                             }
                             if constexpr (CQ && MODE == 1) {
-                                if (qn > 0) drain1(qn);
-                                while (q2 > 0) drain2(min(q2, 64));
+                                // full batches first, then one mixed pass over the rest
+                                while (qn + q2 > 64) {
+                                    if (qn > 0) drain1(min(qn, 64));
+                                    else drain2(64);
+                                }
+                                if (qn + q2 > 0) drain_last();
                             } else if (CQ && !DET && qn > 0) {
                                 drain(qn, std::integral_constant<int, 3>());
                             }
