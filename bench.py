@@ -279,6 +279,14 @@ def gemm_leg(shapes=GEMM_SHAPES, reps=20):
     return out
 
 
+def _progress(msg, t_start=[None]):
+    """One line on stderr when a leg of the benchmark ends."""
+    now = time.perf_counter()
+    if t_start[0] is None:
+        t_start[0] = now
+    print("bench: %s (%.1f s)" % (msg, now - t_start[0]), file=sys.stderr, flush=True)
+
+
 def gemm_tiles_leg(shapes=GEMM_SHAPES + ((4096, 1024, 1728), (8192, 1024, 1024)), reps=20):
     """Every tile of the network GEMM (forced through PIQMC_GEMM_TILE: 1
     64x64, 2 128x64, 3 128x128, 4 and 5 64x64 with K over two or four
@@ -354,6 +362,7 @@ def main():
         return
 
     dev = torch.device("cuda:0")
+    _progress("start")
     L, P, T, S, R = 32, 64, 1.0 / 64, 500, args.replicas
     N, r, c, v = instances.lattice_pm_j(L, 2, np.random.RandomState(2024))
     nbs = instances.neighbors(N, r, c, v)
@@ -438,6 +447,7 @@ def main():
         "residual_per_spin": (e_min - e32 / N) if e32 is not None else None,
         "best_known_energy_per_spin": (e32 / N) if e32 is not None else None,
     }
+    _progress("parallel sweep done")
 
     if not args.no_replay:
         # bit-exact replay on the same instance and replicas, the whole
@@ -463,11 +473,14 @@ def main():
                          "kernel_ms": rk * 1e3, "field_steps": Sr, "replicas": R,
                          "near_ties": int(ties.sum())}
         del perms
+        _progress("replay done")
 
     if not args.no_batch:
         out["batch"] = batch_leg()
+        _progress("batch done")
     if not args.no_c5:
         out["c5"] = c5_leg()
+        _progress("3-D field workload done")
         if args.c5_learned:
             from piqmc import learned
             out["c5_learned"] = learned.run(L=12, dim=3, field_sigma=0.1, P=128, R=1024, hidden=1024,
@@ -475,19 +488,23 @@ def main():
                                             every=4)
     if not args.no_learned:
         out["mfma_gemm"] = gemm_leg()
+        _progress("gemm done")
         from piqmc import learned
         keys = ("workload", "move", "beta", "train_slices", "train_nll_per_epoch", "train_seconds",
                 "train_data_seconds", "train_tflops", "e_ref_per_spin", "e_ref", "curves", "time_ratio")
         e16 = hit_energy("L16_seed1")
         lr = learned.run(eval_steps=(25, 50, 100, 200), e_ref=e16, move="slice")
         out["learned"] = {k: lr[k] for k in keys}
+        _progress("learned slice move done")
         lw = learned.run(eval_steps=(25, 50, 100), e_ref=e16, move="worldline", beta=2.0, rounds=(1, 2, 4, 8))
         out["learned_worldline"] = {k: lw[k] for k in keys}
+        _progress("learned world-line move done")
 
     if not args.no_cpu:
         cb = cpu_baseline(nbs, N, P, np.float32(T), sched)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = value / cb["value"]
+        _progress("cpu baseline done")
         if "replay" in out:
             out["replay"]["speedup_vs_cpu"] = out["replay"]["attempts_per_s"] / cb["value"]
     print(json.dumps(out))
