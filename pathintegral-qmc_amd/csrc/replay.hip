@@ -22,6 +22,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "piqmc_internal.h"
 
@@ -40,81 +41,99 @@ struct ReplayArgs {
     uint32_t *ties;
 };
 
-__device__ __forceinline__ uint32_t getbit(const uint32_t *reg, int L, int lane, int i)
+__device__ __forceinline__ uint32_t getbit(const uint32_t *reg, int Ls, int lane, int i)
 {
-    return (reg[(i >> 5) * L + lane] >> (i & 31)) & 1u;
+    return (reg[(i >> 5) * Ls + lane] >> (i & 31)) & 1u;
 }
 
-// LDS image of a workgroup with L replica slots, words slot-interleaved:
-//   spin regions [nreg][Wn][L] u32, rand() rings [31][L] u32, and with STAGED
-//   (8-byte aligned) the neighbour rows [N][Zp] int2, padded with zero fields
-//   to a multiple of 4 entries, then the sweep's permutations [N][L] u16.
-__host__ __device__ inline size_t replay_words(int nreg, int Wn, int L)
+// LDS word stride of a workgroup with L replica slots: at least 8, so that a
+// row of the bit-packed spins spans a multiple of 32 bytes.
+__host__ __device__ inline int replay_stride(int L) { return L < 8 ? 8 : L; }
+
+// LDS image of a workgroup, words slot-interleaved with stride Ls:
+//   spin regions [nreg][Wn + 1][Ls] u32 (row Wn of each region stays zero),
+//   rand() rings [31][Ls] u32, and with STAGED (8-byte aligned) the neighbour
+//   rows [N][Zp] int2, padded with zero fields to a multiple of 4 entries,
+//   then the sweep's permutations [N][Ls] u16.
+__host__ __device__ inline size_t replay_words(int nreg, int Wn, int Ls)
 {
-    return ((size_t)(nreg * Wn + 31) * L + 1) & ~(size_t)1;
+    return ((size_t)(nreg * (Wn + 1) + 31) * Ls + 1) & ~(size_t)1;
 }
 
-template <bool F64J, bool STAGED>
+template <bool F64J, bool STAGED, bool SA>
 __global__ void replay_kernel(ReplayArgs a)
 {
     extern __shared__ uint32_t sm[];
     // The first Lw lanes of each wave own one replica each ("slots"); a
     // replica's sweep is a dependency chain, so few lanes per wave spread the
     // replicas over every SIMD instead of stacking them in a few waves.
-    const int Lw = a.Lw, L = (blockDim.x >> 6) * Lw;  // L: slots per workgroup (LDS word stride)
-    const int lsh = 31 - __clz(L);                    // L is a power of two
+    const int Lw = a.Lw, L = (blockDim.x >> 6) * Lw;  // L: slots per workgroup
+    const int Ls = replay_stride(L);                  // LDS word stride, a power of two
+    const int lsh = 31 - __clz(Ls);
     const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
     const int lane = wv * Lw + ln;
     const int r = blockIdx.x * L + lane;
     const int N = a.N, P = a.P, Z = a.Z, Wn = a.Wn, R = a.R;
-    const int nreg = a.sa ? 1 : (P == 2 ? 2 : 3);
+    const int nreg = SA ? 1 : (P == 2 ? 2 : 3);
+    const int Wr = Wn + 1;             // rows per spin region (the last one zero)
     const int Zr = STAGED ? a.Zp : Z;  // row stride of the table in use
-    int2 *lrows = reinterpret_cast<int2 *>(sm + replay_words(nreg, Wn, L));
+    int2 *lrows = reinterpret_cast<int2 *>(sm + replay_words(nreg, Wn, Ls));
     uint16_t *lperm = reinterpret_cast<uint16_t *>(lrows + (size_t)N * a.Zp);
     const int2 *rows = STAGED ? lrows : a.rows;
     if (STAGED) {  // every thread helps, then the only barrier
-        // float couplings are staged as -2J (exact): the term -2 s_i s_j J is
-        // then -2J with its sign bit flipped when the two spins differ
         for (int q = threadIdx.x; q < N * a.Zp; q += blockDim.x) {
             const int i = q / a.Zp, z = q - i * a.Zp;
             int2 v = z < Z ? a.rows[(size_t)i * Z + z] : make_int2(i, 0);
-            if (!F64J) v.y = __float_as_int(-2.0f * __int_as_float(v.y));
+            if (!F64J) {
+                // Float couplings are staged as -2J (exact), so the term of an
+                // entry is -2J with its sign bit flipped when the two spins
+                // differ, and the neighbour as the byte offset of its spin
+                // word within a region (a multiple of 32: Ls >= 8) plus
+                // 31 - (j mod 32), the shift that brings its bit to bit 31.
+                // Self entries and padding point at the region's zero row.
+                const int j = v.x;
+                const uint32_t row = j == i ? (uint32_t)Wn : (uint32_t)j >> 5;
+                const uint32_t sh = j == i ? 0u : 31u - ((uint32_t)j & 31u);
+                v.x = (int)(((row << lsh) << 2) | sh);
+                v.y = __float_as_int(-2.0f * __int_as_float(v.y));
+            }
             lrows[q] = v;
         }
         __syncthreads();
     }
     if (ln >= Lw || r >= a.R) return;  // no barriers below: a slot touches only its own words
     uint32_t *cur = sm, *s1, *sp, *ring;
-    if (a.sa) {
+    if (SA) {
         s1 = sp = cur;
-        ring = sm + Wn * L;
+        ring = sm + Wr * Ls;
     } else {
-        s1 = sm + Wn * L;
-        sp = (P == 2) ? s1 : sm + 2 * Wn * L;
-        ring = sm + nreg * Wn * L;  // nreg spin regions, then the rand() rings
+        s1 = sm + Wr * Ls;
+        sp = (P == 2) ? s1 : sm + 2 * Wr * Ls;
+        ring = sm + nreg * Wr * Ls;  // nreg spin regions, then the rand() rings
     }
+    for (int g = 0; g < nreg; ++g) sm[(g * Wr + Wn) * Ls + lane] = 0u;  // zero rows
     auto load = [&](uint32_t *reg, int k) {
-        for (int w = 0; w < Wn; ++w) reg[w * L + lane] = a.bits[((size_t)k * Wn + w) * R + r];
+        for (int w = 0; w < Wn; ++w) reg[w * Ls + lane] = a.bits[((size_t)k * Wn + w) * R + r];
     };
     auto store = [&](const uint32_t *reg, int k) {
-        for (int w = 0; w < Wn; ++w) a.bits[((size_t)k * Wn + w) * R + r] = reg[w * L + lane];
+        for (int w = 0; w < Wn; ++w) a.bits[((size_t)k * Wn + w) * R + r] = reg[w * Ls + lane];
     };
-    if (a.sa) {
+    if (SA) {
         load(cur, 0);
     } else {
         load(s1, 1);
         if (P != 2) load(sp, P - 1);
     }
-    for (int w = 0; w < 31; ++w) ring[w * L + lane] = a.rstate[(size_t)r * 32 + w];
+    for (int w = 0; w < 31; ++w) ring[w * Ls + lane] = a.rstate[(size_t)r * 32 + w];
     uint32_t f = a.rstate[(size_t)r * 32 + 31] % 31u;
     unsigned long long nd = 0;
     uint32_t nt = 0;
     const float T = a.temp;
-    const int nslice = a.sa ? 1 : P;
+    const int nslice = SA ? 1 : P;
 
     for (int s = 0; s < a.S; ++s) {
         const float sv = a.steps[s];
-        const float tdiv = a.sa ? sv : T;
+        const float tdiv = SA ? sv : T;
         const uint32_t svm2 = __float_as_uint(-2.0f * sv);  // exact
         for (int m = 0; m < a.M; ++m) {
             const uint16_t *pg = a.perms + (size_t)(s * a.M + m) * N * R + r;
@@ -128,13 +147,15 @@ __global__ void replay_kernel(ReplayArgs a)
                     for (int q = 0; q < 8; ++q) v[q] = pg[(size_t)min(t0 + q, N - 1) * R];
 #pragma unroll
                     for (int q = 0; q < 8; ++q)
-                        if (t0 + q < N) lperm[(size_t)(t0 + q) * L + lane] = v[q];
+                        if (t0 + q < N) lperm[(size_t)(t0 + q) * Ls + lane] = v[q];
                 }
             }
             for (int k = 0; k < nslice; ++k) {
-                uint32_t *reg = a.sa ? cur : (k == 1 ? s1 : (k == P - 1 ? sp : cur));
-                const bool own = !a.sa && reg == cur;
+                uint32_t *reg = SA ? cur : (k == 1 ? s1 : (k == P - 1 ? sp : cur));
+                const bool own = !SA && reg == cur;
                 if (own) load(cur, k);
+                // this slot's word 0 in the slice's region
+                const char *regb = reinterpret_cast<const char *>(reg + lane);
                 float e = 0.0f;
                 int inext = STAGED ? (int)lperm[lane] : (int)pg[0];
                 // the first four row entries of the next attempt's spin, read
@@ -153,8 +174,8 @@ __global__ void replay_kernel(ReplayArgs a)
                     const uint32_t wo = ((uint32_t)i >> 5) << lsh;
                     uint32_t *wself = reg + wo + lane;
                     const uint32_t ws = *wself;
-                    const uint32_t wl = a.sa ? 0u : sp[wo + lane];  // slice P-1
-                    const uint32_t wr = a.sa ? 0u : s1[wo + lane];  // slice 1
+                    const uint32_t wl = SA ? 0u : sp[wo + lane];  // slice P-1
+                    const uint32_t wr = SA ? 0u : s1[wo + lane];  // slice 1
                     const uint32_t rpi = f >= 3 ? f - 3 : f + 28;
                     const uint32_t rf = ring[(f << lsh) + lane], rr = ring[(rpi << lsh) + lane];
                     const int2 *row = rows + __umul24((uint32_t)i, (uint32_t)Zr);
@@ -165,10 +186,10 @@ __global__ void replay_kernel(ReplayArgs a)
                     }
                     __builtin_amdgcn_sched_barrier(0);
                     const uint32_t bsh = (uint32_t)i & 31u;
-                    const uint32_t bi = 1u << bsh;
+                    const uint32_t ish = ~(uint32_t)i;     // low five bits: 31 - bsh
                     const uint32_t xi = (ws >> bsh) & 1u;  // 1: s_i = -1
                     const float m2 = xi ? 2.0f : -2.0f;    // -2*s_i
-                    if (a.sa) e = 0.0f;
+                    if (SA) e = 0.0f;
                     if constexpr (F64J) {
                         // *_dense: J is read as double, so each term is
                         // added in double and the sum rounded to float
@@ -178,14 +199,20 @@ __global__ void replay_kernel(ReplayArgs a)
                             const int j = row[z].x;
                             const double J = jrow[z];
                             const double t2 = j == i ? (double)m2 * J
-                                                     : (double)m2 * (getbit(reg, L, lane, j) ? -J : J);
+                                                     : (double)m2 * (getbit(reg, Ls, lane, j) ? -J : J);
                             e = (float)((double)e + t2);
                         }
                     } else if constexpr (STAGED) {
-                        // four entries at a time: the spin reads of a group
-                        // issue together; the adds keep row order.  Entry
-                        // (j, -2J): the term is -2J, sign flipped when s_i
-                        // and s_j differ (j == i: when s_i = -1)
+                        // Every term carries the factor s_i, so the sum runs
+                        // on s_i * e: IEEE addition is symmetric under
+                        // negation (fl(-a - b) = -fl(a + b)), so negating the
+                        // running sum, adding the terms without s_i and
+                        // negating back gives the reference's float sum bit
+                        // for bit.  Entry (off | sh, -2J): the term is -2J,
+                        // sign flipped when s_j = -1 (self entries and
+                        // padding read the zero row).
+                        const uint32_t xs = xi << 31;
+                        e = __uint_as_float(__float_as_uint(e) ^ xs);
                         for (int z = 0; z < Zr; z += 4) {
                             if (z) {
 #pragma unroll
@@ -193,14 +220,21 @@ __global__ void replay_kernel(ReplayArgs a)
                             }
                             uint32_t wn[4];
 #pragma unroll
-                            for (int q = 0; q < 4; ++q) wn[q] = reg[(((uint32_t)rw[q].x >> 5) << lsh) + lane];
+                            for (int q = 0; q < 4; ++q)
+                                wn[q] = *reinterpret_cast<const uint32_t *>(regb + ((uint32_t)rw[q].x & ~31u));
                             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                             for (int q = 0; q < 4; ++q) {
-                                const uint32_t bj = rw[q].x == i ? 0u : (wn[q] >> (rw[q].x & 31)) & 1u;
-                                e = e + __uint_as_float((uint32_t)rw[q].y ^ ((xi ^ bj) << 31));
+                                const uint32_t tb = wn[q] << ((uint32_t)rw[q].x & 31u);  // s_j's bit at bit 31
+                                e = e + __uint_as_float((uint32_t)rw[q].y ^ (tb & 0x80000000u));
                             }
                         }
+                        if constexpr (!SA) {
+                            // -2 s_i (J_perp s_l) / s_i = -2 J_perp, sign flipped when s_l = -1
+                            e = e + __uint_as_float(svm2 ^ ((wl << (ish & 31u)) & 0x80000000u));
+                            e = e + __uint_as_float(svm2 ^ ((wr << (ish & 31u)) & 0x80000000u));
+                        }
+                        e = __uint_as_float(__float_as_uint(e) ^ xs);
                     } else {
                         for (int z = 0; z < Z; ++z) {
                             const int2 r1 = row[z];
@@ -208,16 +242,18 @@ __global__ void replay_kernel(ReplayArgs a)
                             if (r1.x == i) {
                                 e = e + m2 * J;
                             } else {
-                                const float sj = getbit(reg, L, lane, r1.x) ? -1.0f : 1.0f;
+                                const float sj = getbit(reg, Ls, lane, r1.x) ? -1.0f : 1.0f;
                                 e = e + m2 * (J * sj);
                             }
                         }
                     }
                     bool flip;
-                    if (!a.sa) {
-                        // -2 s_i (J_perp s_l) = -2 J_perp, sign flipped when s_i != s_l
-                        e = e + __uint_as_float(svm2 ^ ((xi ^ ((wl >> bsh) & 1u)) << 31));
-                        e = e + __uint_as_float(svm2 ^ ((xi ^ ((wr >> bsh) & 1u)) << 31));
+                    if constexpr (!SA) {
+                        if constexpr (F64J || !STAGED) {
+                            // -2 s_i (J_perp s_l) = -2 J_perp, sign flipped when s_i != s_l
+                            e = e + __uint_as_float(svm2 ^ ((xi ^ ((wl >> bsh) & 1u)) << 31));
+                            e = e + __uint_as_float(svm2 ^ ((xi ^ ((wr >> bsh) & 1u)) << 31));
+                        }
                         flip = e > 0.0f;
                     } else {
                         // sa.Anneal accepts ediff >= 0 (piqmc/sa.pyx:114),
@@ -245,19 +281,19 @@ __global__ void replay_kernel(ReplayArgs a)
 #pragma unroll
                         for (int q = 0; q < 4; ++q) rwn[q] = rows[__umul24((uint32_t)inext, (uint32_t)Zr) + q];
                     }
-                    if (flip) *wself = ws ^ bi;
+                    if (flip) *wself = ws ^ (1u << bsh);
                 }
                 if (own) store(cur, k);
             }
         }
     }
-    if (a.sa) {
+    if (SA) {
         store(cur, 0);
     } else {
         store(s1, 1);
         if (P != 2) store(sp, P - 1);
     }
-    for (int w = 0; w < 31; ++w) a.rstate[(size_t)r * 32 + w] = ring[w * L + lane];
+    for (int w = 0; w < 31; ++w) a.rstate[(size_t)r * 32 + w] = ring[w * Ls + lane];
     a.rstate[(size_t)r * 32 + 31] = f;
     a.draws[r] = nd;
     a.ties[r] = nt;
@@ -326,8 +362,9 @@ int run_replay(const piqmc_problem *pb, int R, int P, int S, int M, float temp, 
     // read from global memory.
     const int Zp = (pb->Z + 3) & ~3;
     auto lds_of = [&](int slots, bool staged) {
-        const size_t words = replay_words(nreg, Wn, slots) * 4;
-        return staged ? words + (size_t)N * Zp * sizeof(int2) + (size_t)N * slots * 2 : words;
+        const int Ls = replay_stride(slots);
+        const size_t words = replay_words(nreg, Wn, Ls) * 4;
+        return staged ? words + (size_t)N * Zp * sizeof(int2) + (size_t)N * Ls * 2 : words;
     };
     constexpr size_t kLds = 160 * 1024;
     constexpr int kCUs = 256;
@@ -375,8 +412,12 @@ int run_replay(const piqmc_problem *pb, int R, int P, int S, int M, float temp, 
     a.draws = d_draws;
     a.ties = ties;
     a.jd = pb->dense ? pb->d_jd : nullptr;
-    auto kern = pb->dense ? (staged ? replay_kernel<true, true> : replay_kernel<true, false>)
-                          : (staged ? replay_kernel<false, true> : replay_kernel<false, false>);
+    auto pick = [&](auto SAc) {
+        constexpr bool S_ = decltype(SAc)::value;
+        return pb->dense ? (staged ? replay_kernel<true, true, S_> : replay_kernel<true, false, S_>)
+                         : (staged ? replay_kernel<false, true, S_> : replay_kernel<false, false, S_>);
+    };
+    auto kern = sa ? pick(std::true_type()) : pick(std::false_type());
     (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds(L));
     piqmc::timing_begin(st);
     kern<<<(R + L - 1) / L, 64 * W, lds(L), st>>>(a);
