@@ -334,6 +334,17 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
         }
         const float jp = a.jperp[s];
         const double Ts = (double)a.temps[s];
+        // cached spins with fields: their decision classes of this field step
+        FieldClasses cfc[2][2];
+        if constexpr (FLD && CACHED) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const int q = c == 0 ? (k == 0 ? cq_[0][0] : cq_[0][1]) : (k == 0 ? cq_[1][0] : cq_[1][1]);
+                    field_classes(q >= 0 ? (double)a.hq[q] : 0.0, ZU, jp, Ts, cfc[c][k]);
+                }
+        }
         const bool cq = CQ && un <= 3;  // this step's stochastic words go through the queue
         // CQ kernels run one of three copies of the sweep, chosen per step:
         // MODE 0 without stochastic classes (every stochastic branch compiled
@@ -508,14 +519,16 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                     // for every replica of the workgroup; lanes with act false take
                     // part only in the queue's wave-wide steps
                     auto spin = [&](const bool act, const int q, const int i, const uint32_t info, const int zi,
-                                    const int *nb) __attribute__((always_inline)) {
+                                    const int *nb, const FieldClasses *fcp) __attribute__((always_inline)) {
                             const uint32_t zt = UZ ? zt_u : ((PMJ && !FLD) ? tz[zi] : 0u);
                             // fields: this spin's class thresholds, shared by the replicas below
+                            // (cached spins: computed once per field step)
                             double hfield = 0.0;
                             FieldClasses fc;
                             if (FLD) {
                                 hfield = (double)a.hq[q];
-                                field_classes(hfield, zi, jp, Ts, fc);
+                                if (fcp) fc = *fcp;
+                                else field_classes(hfield, zi, jp, Ts, fc);
                             }
                             for (int rr = 0; rr < nrep; ++rr) {
                               WT qX = 0, qm0 = 0, qm1 = 0, qm2 = 0;  // stochastic word for the queue
@@ -815,7 +828,9 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
     #pragma unroll
                                 for (int z = 0; z < ZC; ++z)
                                     nb[z] = c0k ? (k0k ? cN[0][0][z] : cN[0][1][z]) : (k0k ? cN[1][0][z] : cN[1][1][z]);
-                                spin(q >= 0, q, i, info, ZU, nb);
+                                const FieldClasses *fcp = nullptr;
+                                if constexpr (FLD) fcp = c0k ? (k0k ? &cfc[0][0] : &cfc[0][1]) : (k0k ? &cfc[1][0] : &cfc[1][1]);
+                                spin(q >= 0, q, i, info, ZU, nb, fcp);
                             }
                             if constexpr (CQ && MODE == 1) {
                                 // full batches first, then one mixed pass over the rest
@@ -853,7 +868,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                             if (z < a.zmax) nb[z] = z < zi ? a.cnb[(size_t)z * N + q] : i;
                                     }
                                 }
-                                spin(act, q, i, info, zi, nb);
+                                spin(act, q, i, info, zi, nb, nullptr);
                             }
                             if constexpr (CQ && MODE == 1) {
                                 // full batches first, then one mixed pass over the rest
@@ -919,6 +934,16 @@ __global__ void __attribute__((flatten)) colored_kernel(ColArgs<WT> a)
     colored_body<WT, PMJ, UZ, FLD, ZU, CACHED, CQ>(a, r0, min(a.rpw, a.R - r0), smem);
 }
 
+// The field sweep with cached spins and decision classes (cubic lattices): at
+// most 512 threads, so that the cached registers fit without spilling.
+template <typename WT>
+__global__ void __attribute__((flatten)) __launch_bounds__(512) colored_fields_cached_kernel(ColArgs<WT> a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int r0 = blockIdx.x * a.rpw;
+    colored_body<WT, true, false, true, 6, true, false>(a, r0, min(a.rpw, a.R - r0), smem);
+}
+
 // The queue kernels (+-1 couplings, uniform degree) with a register budget of
 // five wavefronts per SIMD: left to itself the compiler takes 117 registers
 // and four; on the 1024-spin workload five run 15.1 -> 14.4 ms despite a
@@ -972,6 +997,14 @@ inline bool queue_enabled()
 inline bool dense_wg_enabled()
 {
     const char *e = std::getenv("PIQMC_DENSE_WG");
+    return !(e && e[0] == '0');
+}
+
+// The cached-register field sweep (PIQMC_CACHED_FIELDS=0 selects the general
+// one, for comparisons).
+inline bool cached_fields_enabled()
+{
+    const char *e = std::getenv("PIQMC_CACHED_FIELDS");
     return !(e && e[0] == '0');
 }
 
@@ -2250,8 +2283,19 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
     piqmc::timing_begin(st);
     if (!pmj && pb->pmjf) {
         a.zuni = -1;
-        (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        colored_kernel<WT, true, false, true><<<grid, block, lds, st>>>(a);
+        int biggest = 0;
+        for (int c = 0; c < pb->ncolors; ++c) biggest = std::max(biggest, pb->color_off[c + 1] - pb->color_off[c]);
+        // two colors of at most two spins per thread and six couplings per
+        // spin besides the field (cubic lattices): spins, neighbours and
+        // decision classes cached in registers
+        if (pb->ncolors == 2 && biggest <= 2 * block && block <= 512 && pb->zuni_nf == 6 && cached_fields_enabled()) {
+            (void)hipFuncSetAttribute((const void *)colored_fields_cached_kernel<WT>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            colored_fields_cached_kernel<WT><<<grid, block, lds, st>>>(a);
+        } else {
+            (void)hipFuncSetAttribute((const void *)colored_kernel<WT, true, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            colored_kernel<WT, true, false, true><<<grid, block, lds, st>>>(a);
+        }
     } else if (pmj) {
         if (a.zuni == 4) {
             launch_uniform<WT, 4>(a, pb, grid, block, lds, st);

@@ -404,6 +404,15 @@ extern "C" int piqmc_problem_create(int N, int Z, const double *nbs, piqmc_probl
             if (p->J[(size_t)i * Z + e] != 0.0f) ++z;
         if (z != p->zmax) p->zuni = -1;
     }
+    p->zuni_nf = -2;
+    for (int i = 0; i < N && p->zuni_nf != -1; ++i) {
+        int z = 0;
+        for (int e = 0; e < Z; ++e)
+            if (p->J[(size_t)i * Z + e] != 0.0f && p->idx[(size_t)i * Z + e] != i) ++z;
+        if (p->zuni_nf == -2) p->zuni_nf = z;
+        else if (z != p->zuni_nf) p->zuni_nf = -1;
+    }
+    if (p->zuni_nf < 0) p->zuni_nf = -1;
     std::vector<int2> rows((size_t)N * Z);
     for (size_t t = 0; t < rows.size(); ++t) {
         rows[t].x = p->idx[t];

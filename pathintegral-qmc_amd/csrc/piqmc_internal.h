@@ -48,6 +48,7 @@ struct piqmc_problem {
     int pmj = 0;
     int zmax = 0;                 // max nonzero off-diagonal entries per row
     int zuni = -1;                // common degree when every row has zmax entries, else -1
+    int zuni_nf = -1;             // common number of non-field couplings per row, else -1
     int sweep_path = 0;           // parallel sweep: 0 auto, 1 word per thread, 2 dense (lane per slice)
     int dense_variant = 0;        // dense sweep layout: 0 auto, 1..5 (piqmc_problem_set_dense_variant)
     int32_t *d_cnb = nullptr;     // [zmax][N] neighbour of position q (color order)

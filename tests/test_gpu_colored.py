@@ -57,11 +57,15 @@ def test_colored_3d_pm_j(gpu):
     assert np.array_equal(got, ref) and np.array_equal(etg, etr)
 
 
+@pytest.mark.parametrize("cached", ["1", "0"])
 @pytest.mark.parametrize("P,T,S", [(16, 0.05, 15), (128, 1 / 128, 12), (8, 2.0, 10), (32, 0.5, 10)])
-def test_colored_fields_path(gpu, P, T, S):
+def test_colored_fields_path(gpu, monkeypatch, P, T, S, cached):
     # 3-D +-J with Gaussian fields (the large sparse workload's structure): the
     # bit-sliced path with per-spin thresholds; T=2 and 0.5 give many
-    # stochastic classes per spin (the per-bit fallback)
+    # stochastic classes per spin (the per-bit fallback).  Both the sweep that
+    # caches spins, neighbours and decision classes in registers (six
+    # couplings per spin, two colors) and the general one.
+    monkeypatch.setenv("PIQMC_CACHED_FIELDS", cached)
     N, nbs, _, _, _ = pm_j_problem(4, dim=3, seed=4 + P, field_sigma=0.1)
     pb, got, ref, etg, etr = _run(gpu, nbs, P, np.linspace(3.0, 1e-8, S), 1, T, R=3, seed=8 + P,
                                   rbase=2, sbase=5)
