@@ -1754,10 +1754,33 @@ __global__ void __launch_bounds__(kDenseMaxBlock) dense_split_kernel(ColArgs<WT>
 // at a stride of NP doubles, an update writes consecutive spins of one slice,
 // and neither conflicts on the banks.
 __host__ __device__ inline int fields_pitch(int N) { return N | 1; }
+// Rows the cached-field update covers: four entries per lane of a wavefront.
+constexpr int kFieldsMaxRow = 256;
 __host__ __device__ inline size_t dense_fields_bytes(int N, int W, int word_bytes)
 {
     const size_t state = ((size_t)2 * N * word_bytes + 7) & ~(size_t)7;
     return state + (size_t)W * fields_pitch(N) * sizeof(double) + (((size_t)N * sizeof(int) + 7) & ~(size_t)7);
+}
+
+// Wavefronts that share one replica of the cached-field sweep.  Standalone
+// (waves_per_wg 0): four (or two) when only one replica's tables fit a
+// workgroup's LDS.  In a workgroup of waves_per_wg wavefronts holding
+// replicas_per_wg replicas: waves_per_wg / replicas_per_wg when that divides
+// evenly and splits the slices evenly.  1: one wavefront per replica
+// (PIQMC_FIELDS_SPLIT=0 forces it).
+inline int fields_split_waves(int W, int replicas_per_wg, int waves_per_wg = 0)
+{
+    const char *e = std::getenv("PIQMC_FIELDS_SPLIT");
+    if (e && e[0] == '0') return 1;
+    if (waves_per_wg > 0) {
+        if (replicas_per_wg < 1 || waves_per_wg % replicas_per_wg) return 1;
+        const int nw = waves_per_wg / replicas_per_wg;
+        return nw >= 2 && W % nw == 0 && W / nw >= 2 ? nw : 1;
+    }
+    if (replicas_per_wg > 1) return 1;
+    if (W % 4 == 0 && W >= 8) return 4;
+    if (W % 2 == 0 && W >= 4) return 2;
+    return 1;
 }
 
 // Initial local fields of the cached-field sweep, one thread per (replica,
@@ -1896,6 +1919,134 @@ __device__ void dense_fields_replica(const ColArgs<WT> &a, int r, unsigned char 
     for (int q = lane; q < 2 * N; q += 64) gst[q] = st[q];
 }
 
+// One replica over the nw wavefronts of a workgroup: wavefront wv owns slices
+// [wv*Ws, (wv+1)*Ws), Ws = W/nw, and their rows of the field table (the
+// layout of dense_fields_replica: the table's W rows split among the
+// wavefronts).  Slices of one parity do not interact within a half sweep, so
+// the wavefronts decide their slices of every spin independently: each reads
+// the shared state word (only its own bits matter to it), XORs its flips in
+// atomically, and updates its own rows.  They meet at a workgroup barrier
+// between half sweeps, where the other parity becomes the Trotter input.
+// Every wavefront draws the word's Philox blocks itself (the counter depends on
+// the word, not the slice), so the decisions equal the one-wavefront sweep's.
+// Used where one replica's tables fill the LDS and the other wavefronts of the
+// workgroup would idle: the chain of N decisions per half sweep, which sets a
+// lone replica's time, is then shared out.
+template <typename WT>
+__device__ void dense_fields_split(const ColArgs<WT> &a, int r, unsigned char *base, int wv, int nw)
+{
+    const int N = a.N, W = a.W, Z = a.Z, lane = threadIdx.x & 63, NP = fields_pitch(N);
+    const int Ws = W / nw, s0 = wv * Ws;
+    WT *st = reinterpret_cast<WT *>(base);
+    double *tab0 = reinterpret_cast<double *>(base + (((size_t)2 * N * sizeof(WT) + 7) & ~(size_t)7));
+    double *tab = tab0 + (size_t)s0 * NP;  // this wavefront's rows
+    const WT one = WT(1);
+    const int ll = lane < Ws ? lane : 0;  // local slice
+    const int sl = lane < Ws ? s0 + lane : W;  // global slice (W: inactive lane)
+    const uint32_t rep = a.replica_base + (uint32_t)r;
+    WT *gst = a.packed + (size_t)r * 2 * N;
+    double *gt = a.fields + (size_t)r * 2 * W * NP;  // [2][W][NP]
+    int *ord = reinterpret_cast<int *>(tab0 + (size_t)W * NP);
+    const int tid = wv * 64 + lane, bd = nw * 64;  // this replica's wavefronts (a workgroup may hold several)
+    for (int q = tid; q < N; q += bd) ord[q] = a.order[q];
+    for (int q = tid; q < 2 * N; q += bd) st[q] = gst[q];
+    __syncthreads();
+    constexpr int kPre = 4;  // row entries per lane held in registers (rows up to 256)
+    int2 pre[kPre], cur[kPre];
+    auto fetch_row = [&](int i) {
+        const int2 *row = a.rows + (size_t)i * Z;
+#pragma unroll
+        for (int k = 0; k < kPre; ++k) pre[k] = row[min(lane + 64 * k, Z - 1)];
+    };
+    for (int s = 0; s < a.S; ++s) {
+        const float jp = a.jperp[s];
+        const double Ts = (double)a.temps[s];
+        for (int m = 0; m < a.M; ++m) {
+            const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
+            for (int tau = 0; tau < 2; ++tau) {
+                WT *mine = st + tau * N;
+                const WT *other = st + (1 - tau) * N;
+                double *gtt = gt + (size_t)tau * W * NP + (size_t)s0 * NP;
+                for (int t = lane; t < Ws * NP; t += 64) tab[t] = gtt[t];
+                fetch_row(__builtin_amdgcn_readfirstlane(ord[0]));
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (int q = 0; q < N; ++q) {
+                    const int i = __builtin_amdgcn_readfirstlane(ord[q]);
+#pragma unroll
+                    for (int k = 0; k < kPre; ++k) cur[k] = pre[k];
+                    if (q + 1 < N) fetch_row(__builtin_amdgcn_readfirstlane(ord[q + 1]));
+                    const double h = tab[(size_t)ll * NP + i];
+                    const WT S_ = mine[i], O = other[i];
+                    // the ballot is over lanes: lane l decided slice s0 + l
+                    const WT bal = dense_decide<WT, true>(a, h, S_, O, sl, tau, jp, Ts, g, i, rep) << s0;
+                    if (lane == 0 && bal) atomicXor(&mine[i], bal);
+                    constexpr int kFl = 4;
+                    int jj[kPre];
+                    double jv[kPre];
+#pragma unroll
+                    for (int k = 0; k < kPre; ++k) {
+                        const float J = __int_as_float(cur[k].y);
+                        const bool use = lane + 64 * k < Z && cur[k].x != i && J != 0.0f;
+                        jj[k] = use ? cur[k].x : -1;
+                        jv[k] = 2.0 * (double)J;
+                    }
+                    WT rem = bal;
+                    while (rem) {
+                        int mm[kFl];
+                        double sg[kFl];
+#pragma unroll
+                        for (int f = 0; f < kFl; ++f) {
+                            mm[f] = rem ? ctz(rem) : -1;
+                            sg[f] = mm[f] >= 0 && ((S_ >> mm[f]) & one) ? 1.0 : -1.0;
+                            rem &= rem - (rem ? WT(1) : WT(0));
+                        }
+                        double v[kFl][kPre];
+#pragma unroll
+                        for (int f = 0; f < kFl; ++f)
+#pragma unroll
+                            for (int k = 0; k < kPre; ++k)
+                                v[f][k] = (mm[f] >= 0 && jj[k] >= 0) ? tab[(size_t)(mm[f] - s0) * NP + jj[k]] : 0.0;
+#pragma unroll
+                        for (int f = 0; f < kFl; ++f)
+#pragma unroll
+                            for (int k = 0; k < kPre; ++k)
+                                if (mm[f] >= 0 && jj[k] >= 0)
+                                    tab[(size_t)(mm[f] - s0) * NP + jj[k]] = v[f][k] + sg[f] * jv[k];
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+                for (int t = lane; t < Ws * NP; t += 64) gtt[t] = tab[t];
+                __syncthreads();  // this parity complete: the Trotter input of the next half sweep
+            }
+        }
+        if (a.etrace) {
+            // wavefront 0 sums the energy; the others wait before changing the state
+            if (wv == 0) {
+                double acc = 0.0;
+                for (int i = lane; i < N; i += 64) {
+                    const WT a0 = st[i], a1 = st[N + i];
+                    const int2 *row = a.rows + (size_t)i * a.Z;
+                    for (int e = 0; e < a.Z; ++e) {
+                        const int2 rw = row[e];
+                        const double J = (double)__int_as_float(rw.y);
+                        if (J == 0.0) continue;
+                        if (rw.x == i) acc -= J * (double)(a.P - 2 * (popc(a0) + popc(a1)));
+                        else acc -= 0.5 * J * (double)(a.P - 2 * (popc(a0 ^ st[rw.x]) + popc(a1 ^ st[N + rw.x])));
+                    }
+                }
+                acc = wave_sum(acc);
+                if (lane == 0) a.etrace[(size_t)r * a.S + s] = acc / (double)a.P;
+            }
+            __syncthreads();
+        }
+    }
+    for (int q = tid; q < 2 * N; q += bd) gst[q] = st[q];
+}
+
 // One replica per wavefront; as many wavefronts per workgroup as the launch
 // gives (the host keeps the LDS of all of them within 160 KiB).
 template <typename WT>
@@ -1905,6 +2056,14 @@ __global__ void __launch_bounds__(256) dense_fields_kernel(ColArgs<WT> a)
     const int w = threadIdx.x >> 6, r = blockIdx.x * (blockDim.x >> 6) + w;
     if (r >= a.R) return;
     dense_fields_replica<WT>(a, r, smem + (size_t)w * dense_fields_bytes(a.N, a.W, sizeof(WT)));
+}
+
+// One replica per workgroup, its slices split over the workgroup's wavefronts.
+template <typename WT>
+__global__ void __launch_bounds__(256) dense_fields_split_kernel(ColArgs<WT> a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    dense_fields_split<WT>(a, blockIdx.x, smem, threadIdx.x >> 6, blockDim.x >> 6);
 }
 
 // int8 [R][P][N] -> packed [R][2][N] (bit m of parity tau = slice 2m+tau, set = -1)
@@ -2112,7 +2271,8 @@ size_t colored_scratch_bytes(const piqmc_problem *pb, int R, int P, int S)
 {
     const size_t wb = P / 2 <= 32 ? 4 : 8;
     size_t b = CallBuffers::round_up(sizeof(float) * (S ? 2 * S : 1));
-    if (use_dense(pb) && (pb->dense_variant == 6 || (pb->dense_variant == 0 && pb->exact_rows && pb->fields_ok)))
+    if (use_dense(pb) && pb->Z <= kFieldsMaxRow &&
+        (pb->dense_variant == 6 || (pb->dense_variant == 0 && pb->exact_rows && pb->fields_ok)))
         b += CallBuffers::round_up(sizeof(double) * 2 * (size_t)(P / 2) * (pb->N | 1) * R);
     if (!use_dense(pb)) {
         if (pb->pmj && S)
@@ -2152,8 +2312,8 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         if (pb->dense_variant == 0) {
             // cached fields while replicas are few (256-spin SK, P=128: 33 ms
             // against 46 ms pipelined at 4 replicas, 20 against 18 at 64)
-            fields_auto = pb->exact_rows && pb->fields_ok && R <= 32 && dense_wg_enabled() &&
-                          dense_fields_bytes(N, W, sizeof(WT)) <= 160 * 1024;
+            fields_auto = pb->exact_rows && pb->fields_ok && pb->Z <= kFieldsMaxRow && R <= 32 &&
+                          dense_wg_enabled() && dense_fields_bytes(N, W, sizeof(WT)) <= 160 * 1024;
             // a workgroup per replica while replicas are few (1.2-1.9x up to 1024
             // 256-spin replicas); from 4096 on, a wavefront each is 1.3x faster
             const bool wg0 = !fields_auto && pb->exact_rows && dense_wg_enabled() && R < 2048;
@@ -2168,10 +2328,11 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
                                                       "rows, P/2 dividing 16 and block <= 256");
             split = true;
         } else if (pb->dense_variant == 6) {
-            if (!pb->exact_rows || !pb->fields_ok || dense_fields_bytes(N, W, sizeof(WT)) > 160 * 1024)
+            if (!pb->exact_rows || !pb->fields_ok || pb->Z > kFieldsMaxRow ||
+                dense_fields_bytes(N, W, sizeof(WT)) > 160 * 1024)
                 return piqmc::set_error(PIQMC_EINVAL, "dense variant 6 (cached fields) needs exactly summable rows "
-                                                      "that list each neighbour once with the same coupling both "
-                                                      "ways, and W x N fields that fit the LDS");
+                                                      "of at most 256 entries that list each neighbour once with the "
+                                                      "same coupling both ways, and W x N fields that fit the LDS");
         } else if (pb->dense_variant >= 3) {
             pipe = pb->dense_variant >= 4;
             pw = pb->dense_variant == 5 ? 8 : 4;
@@ -2205,10 +2366,18 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
             // one replica per wavefront, as many per workgroup as block and LDS allow
             const size_t per = dense_fields_bytes(N, W, sizeof(WT));
             const int wf = std::max(1, std::min(std::min(wpb, 4), (int)((160 * 1024) / per)));
-            (void)hipFuncSetAttribute((const void *)dense_fields_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)(per * wf));
             fields_init_kernel<WT><<<grid_for((size_t)R * 2 * W * N, 256), 256, 0, st>>>(a);
-            dense_fields_kernel<WT><<<(R + wf - 1) / wf, 64 * wf, per * wf, st>>>(a);
+            const int ns = fields_split_waves(W, wf);
+            if (ns > 1) {
+                // one replica's tables fill the LDS: its slices over ns wavefronts
+                (void)hipFuncSetAttribute((const void *)dense_fields_split_kernel<WT>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)per);
+                dense_fields_split_kernel<WT><<<R, 64 * ns, per, st>>>(a);
+            } else {
+                (void)hipFuncSetAttribute((const void *)dense_fields_kernel<WT>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)(per * wf));
+                dense_fields_kernel<WT><<<(R + wf - 1) / wf, 64 * wf, per * wf, st>>>(a);
+            }
         } else if (wg && pipe) {
             // one replica per workgroup, one spin decided while the next is summed
             (void)hipFuncSetAttribute((const void *)dense_pipe_kernel<WT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lwg);

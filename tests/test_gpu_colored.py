@@ -458,3 +458,20 @@ def test_autotuned_batch_equals_individual_runs(gpu):
         pb.qa_colored(ref, sched, 1, 1.0 / P, 500 + t, replica_base=7 * t)
         torch.cuda.synchronize()
         assert torch.equal(c8, ref), t
+
+
+@pytest.mark.parametrize("split", ["1", "0"])
+def test_cached_fields_split_over_wavefronts(gpu, monkeypatch, split):
+    # 256-spin SK at P=128: one replica's field tables fill a workgroup's LDS,
+    # so its 64 slices of each parity are shared out over four wavefronts
+    # (PIQMC_FIELDS_SPLIT=0: one wavefront per replica); both give the CPU
+    # twin's spins and energies
+    from piqmc import instances
+    monkeypatch.setenv("PIQMC_FIELDS_SPLIT", split)
+    n, r, c, v = instances.sk(256, np.random.RandomState(4))
+    nbs = instances.neighbors(n, r, c, v)
+    pb, got, ref, etg, etr = _run(gpu, nbs, 128, np.linspace(3.0, 1e-8, 6), 1, 1 / 128, R=2, seed=17,
+                                  rbase=3, sbase=1)
+    assert pb.exact_rows
+    assert np.array_equal(got, ref)
+    np.testing.assert_allclose(etg, etr, rtol=1e-12, atol=1e-12)
