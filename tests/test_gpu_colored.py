@@ -475,3 +475,19 @@ def test_cached_fields_split_over_wavefronts(gpu, monkeypatch, split):
     assert pb.exact_rows
     assert np.array_equal(got, ref)
     np.testing.assert_allclose(etg, etr, rtol=1e-12, atol=1e-12)
+
+
+def test_cached_fields_not_used_for_rows_over_256(gpu):
+    # the cached-field update covers rows of up to 256 entries: a 300-spin SK
+    # glass (299 couplings per row) takes another dense layout automatically
+    # and gives the CPU twin's spins; asking for cached fields is refused
+    import torch
+    from piqmc import device, instances
+    n, r, c, v = instances.sk(300, np.random.RandomState(2))
+    nbs = instances.neighbors(n, r, c, v)
+    pb, got, ref, etg, etr = _run(gpu, nbs, 16, np.linspace(3.0, 1e-8, 5), 1, 1 / 16, R=2, seed=23)
+    assert np.array_equal(got, ref)
+    pb.set_dense_variant("fields")
+    c8 = torch.ones((1, 16, n), dtype=torch.int8, device=gpu)
+    with pytest.raises(ValueError):
+        pb.qa_colored(c8, np.linspace(3.0, 1e-8, 3), 1, 1 / 16, 1)
