@@ -353,8 +353,11 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
         auto sweeps = [&](auto MODEc) __attribute__((always_inline)) {
             constexpr int MODE = decltype(MODEc)::value;
             constexpr bool DET = MODE == 0;
-            const bool one = MODE == 1 || (MODE == 2 && un == 1);  // a single stochastic class
-            const bool cqm = MODE == 1 || cq;
+            // MODE 1, 3, 4: one stochastic class, in Trotter class u_t = 0, 1, 2
+            constexpr bool SINGLE = MODE == 1 || MODE == 3 || MODE == 4;
+            constexpr int UTC = MODE == 3 ? 1 : (MODE == 4 ? 2 : 0);
+            const bool one = SINGLE || (MODE == 2 && un == 1);  // a single stochastic class
+            const bool cqm = SINGLE || cq;
             for (int m = 0; m < a.M; ++m) {
                 const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
                 for (int tau = 0; tau < 2; ++tau) {
@@ -608,7 +611,15 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                         ge[7] = c2 & c1 & c0;
                                         ge[8] = WT(0);
                                         flips = (e0 & ge[ka0]) | (e1 & ge[ka1]) | (e2 & ge[ka2]);
-                                        X = DET ? WT(0) : ((e0 & ge[ks0]) | (e1 & ge[ks1]) | (e2 & ge[ks2]));
+                                        if constexpr (SINGLE) {
+                                            // the step's one stochastic class (u_t, u_s): exactly the
+                                            // bits of Trotter class u_t with count u_s (no accepted
+                                            // bit is among them, so no mask by ~flips)
+                                            const WT eu = UTC == 0 ? e0 : (UTC == 1 ? e1 : e2);
+                                            X = eu & ge[us_s[0]] & ~ge[us_s[0] + 1];
+                                        } else {
+                                            X = DET ? WT(0) : ((e0 & ge[ks0]) | (e1 & ge[ks1]) | (e2 & ge[ks2]));
+                                        }
                                     } else {
                                         flips = (e0 & ge_pick((int)(zt & 15), c0, c1, c2)) |
                                                 (e1 & ge_pick((int)((zt >> 8) & 15), c0, c1, c2)) |
@@ -617,7 +628,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                             (e1 & ge_pick((int)((zt >> 12) & 15), c0, c1, c2)) |
                                             (e2 & ge_pick((int)((zt >> 20) & 15), c0, c1, c2));
                                     }
-                                    X &= ~flips;
+                                    if constexpr (!SINGLE) X &= ~flips;
                                     if (CQ && cqm) {
                                         // masks of the step's uniform classes, queued with the word
                                         // (a single class holds every stochastic bit)
@@ -798,7 +809,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                           const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi(
                                                                    (uint32_t)(bal >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                                          if (MODE == 1)
+                                          if (SINGLE)
                                               wq1[pos] = make_uint2((uint32_t)i | ((uint32_t)rr << 16), (uint32_t)qm0);
                                           else
                                               wq[pos] = make_uint4((uint32_t)i | ((uint32_t)rr << 16), (uint32_t)qm0,
@@ -806,7 +817,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                       }
                                       qn += (int)__popcll(bal);
                                       if (qn >= 64) {
-                                          if constexpr (MODE == 1) drain1(64);
+                                          if constexpr (SINGLE) drain1(64);
                                           else drain(64, std::integral_constant<int, 3>());
                                       }
                                   }
@@ -832,7 +843,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 if constexpr (FLD) fcp = c0k ? (k0k ? &cfc[0][0] : &cfc[0][1]) : (k0k ? &cfc[1][0] : &cfc[1][1]);
                                 spin(q >= 0, q, i, info, ZU, nb, fcp);
                             }
-                            if constexpr (CQ && MODE == 1) {
+                            if constexpr (CQ && SINGLE) {
                                 // full batches first, then one mixed pass over the rest
                                 while (qn + q2 > 64) {
                                     if (qn > 0) drain1(min(qn, 64));
@@ -870,7 +881,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 }
                                 spin(act, q, i, info, zi, nb, nullptr);
                             }
-                            if constexpr (CQ && MODE == 1) {
+                            if constexpr (CQ && SINGLE) {
                                 // full batches first, then one mixed pass over the rest
                                 while (qn + q2 > 64) {
                                     if (qn > 0) drain1(min(qn, 64));
@@ -888,7 +899,9 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
         };
         if constexpr (CQ) {
             if (un == 0) sweeps(std::integral_constant<int, 0>());
-            else if (un == 1) sweeps(std::integral_constant<int, 1>());
+            else if (un == 1 && ut_s[0] == 0) sweeps(std::integral_constant<int, 1>());
+            else if (un == 1 && ut_s[0] == 1) sweeps(std::integral_constant<int, 3>());
+            else if (un == 1) sweeps(std::integral_constant<int, 4>());
             else sweeps(std::integral_constant<int, 2>());
         } else {
             sweeps(std::integral_constant<int, 2>());
