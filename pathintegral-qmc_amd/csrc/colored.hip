@@ -153,23 +153,12 @@ __device__ __forceinline__ WT ge_pick(int k, WT c0, WT c1, WT c2)
     }
 }
 
-// "count >= k" for a per-lane k (0..8) without branches: k is a nibble of a
-// packed word, its bits become all-ones / zero masks (signed one-bit field
-// extracts) compared most significant bit first.
+// Bit `bit` of a packed per-lane word as an all-ones or zero mask (a signed
+// one-bit field extract).
 template <typename WT>
 __device__ __forceinline__ WT bitmask(uint32_t pk, int bit)
 {
     return (WT)(int64_t)((int32_t)(pk << (31 - bit)) >> 31);  // 0 or all ones
-}
-template <typename WT>
-__device__ __forceinline__ WT ge_nib(uint32_t pk, int nib, WT c0, WT c1, WT c2)
-{
-    const int b = 4 * nib;
-    const WT K1 = bitmask<WT>(pk, b + 1), K2 = bitmask<WT>(pk, b + 2);
-    const WT t0 = c0 | ~bitmask<WT>(pk, b);
-    const WT t1 = (c1 & t0) | (~K1 & (c1 | t0));
-    const WT t2 = (c2 & t1) | (~K2 & (c2 | t1));
-    return t2 & ~bitmask<WT>(pk, b + 3);
 }
 
 // Decision classes of one spin with a field h in one field step, for +-1
@@ -588,16 +577,33 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                     WT X;
                                     WT ge[KMAX + 1];  // "count >= k" masks (uniform-degree path)
                                     if (FLD) {
-                                        // per-spin thresholds for (sigma, u_t): sigma = +1 bits are ~S_
-                                        flips = 0;
-                                        X = 0;
-    #pragma unroll
-                                        for (int cb = 0; cb < 6; ++cb) {
-                                            const WT eu = (cb % 3) == 0 ? e0 : ((cb % 3) == 1 ? e1 : e2);
-                                            const WT m = (cb < 3 ? ~S_ : S_) & eu;
-                                            flips |= m & ge_nib<WT>(fc.ka, cb, c0, c1, c2);
-                                            X |= m & ge_nib<WT>(fc.ks, cb, c0, c1, c2);
-                                        }
+                                        // Per-spin thresholds for (sigma, u_t), nibble sigma*3 + u_t
+                                        // (sigma = +1 bits are ~S_).  Each bit's threshold is built as
+                                        // bit planes by selects on (S_, uL, uR) -- u_t = 1 where uL
+                                        // and uR differ, else 2 or 0 by uL -- and compared with the
+                                        // bond count once: the same masks as a "count >= k" per
+                                        // (sigma, u_t) class, in a third of the work.
+                                        const WT xr = uL ^ uR;
+                                        auto kplane = [&](uint32_t pk, int j) __attribute__((always_inline)) -> WT {
+                                            const WT a0 = bitmask<WT>(pk, j), a1 = bitmask<WT>(pk, 4 + j),
+                                                     a2 = bitmask<WT>(pk, 8 + j);
+                                            const WT b0 = bitmask<WT>(pk, 12 + j), b1 = bitmask<WT>(pk, 16 + j),
+                                                     b2 = bitmask<WT>(pk, 20 + j);
+                                            const WT fp = (xr & a1) | (~xr & ((uL & a2) | (~uL & a0)));
+                                            const WT fn = (xr & b1) | (~xr & ((uL & b2) | (~uL & b0)));
+                                            return (S_ & fn) | (~S_ & fp);
+                                        };
+                                        auto ge_k = [&](uint32_t pk) __attribute__((always_inline)) -> WT {
+                                            const WT K0 = kplane(pk, 0), K1 = kplane(pk, 1), K2 = kplane(pk, 2);
+                                            WT t = c0 | ~K0;
+                                            t = (c1 & ~K1) | (~(c1 ^ K1) & t);
+                                            t = (c2 & ~K2) | (~(c2 ^ K2) & t);
+                                            // thresholds reach 8 ("never") only with seven couplings
+                                            if constexpr (!(ZU > 0 && ZU <= 6)) t &= ~kplane(pk, 3);
+                                            return t;
+                                        };
+                                        flips = ge_k(fc.ka) & valid;
+                                        X = ge_k(fc.ks) & valid;
                                     } else if (UZ) {
                                         // thresholds are wave-uniform: every "count >= k" mask
                                         // once, then six selections by a uniform index
@@ -678,11 +684,12 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                             for (int t = 0; t < 3; ++t) {
                                                 if (t < ns) {
                                                     const uint32_t sb = (fc.slot >> (8 * t)) & 255u;
-                                                    const int cb = (int)(sb & 7u), us = (int)(sb >> 4);
+                                                    const int cb = (int)(sb & 7u);
                                                     const WT eu = (cb % 3) == 0 ? e0 : ((cb % 3) == 1 ? e1 : e2);
-                                                    const uint32_t kk = (uint32_t)us | ((uint32_t)(us + 1) << 4);
-                                                    const WT mk = (cb < 3 ? ~S_ : S_) & eu & ge_nib<WT>(kk, 0, c0, c1, c2) &
-                                                                  ~ge_nib<WT>(kk, 1, c0, c1, c2);
+                                                    // count == u_s (u_s <= 7: three bits at 4..6 of the slot)
+                                                    const WT eq = ~((c0 ^ bitmask<WT>(sb, 4)) | (c1 ^ bitmask<WT>(sb, 5)) |
+                                                                    (c2 ^ bitmask<WT>(sb, 6)));
+                                                    const WT mk = (cb < 3 ? ~S_ : S_) & eu & eq;
                                                     if (t == 0) { cm0 = mk; ct0 = fc.st[0]; }
                                                     else if (t == 1) { cm1 = mk; ct1 = fc.st[1]; }
                                                     else { cm2 = mk; ct2 = fc.st[2]; }
