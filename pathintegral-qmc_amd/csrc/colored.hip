@@ -195,14 +195,18 @@ __device__ __forceinline__ void field_classes(double h, int z, float jp, double 
         const int sg = cb / 3, ut = cb % 3;
         const double hs = sg ? -h : h;
         const double B = (double)jp * (double)(2 - 2 * ut);
-        int amin = z + 1, smin, us = z;
+        // Accepted counts are those with X(u_s) = (hs + (z - 2 u_s)) + B < 0, and X
+        // does not increase with u_s: the boundary is estimated in closed form
+        // and then checked on the exact expression (rounding moves it by one
+        // at most), instead of walking down from u_s = z.
+        auto Xat = [&](int u) { return (hs + (double)(z - 2 * u)) + B; };
+        const double est = fmin(fmax((hs + B + (double)z) * 0.5, -1.0), (double)z + 1.0);
+        int amin = min(max((int)floor(est) + 1, 0), z + 1);
+        while (amin <= z && !(Xat(amin) < 0.0)) ++amin;
+        while (amin > 0 && Xat(amin - 1) < 0.0) --amin;
+        int smin, us = amin - 1;
         unsigned int thr = 0;
-        int cls = 0;
-        for (; us >= 0; --us) {
-            cls = field_class(hs, z, us, B, T, &thr);
-            if (cls != 1) break;
-            amin = us;
-        }
+        int cls = us >= 0 ? field_class(hs, z, us, B, T, &thr) : 1;
         smin = amin;
         for (; us >= 0 && cls == 2; --us) {
             smin = us;
