@@ -353,10 +353,9 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
             const bool cqm = SINGLE || cq;
             for (int m = 0; m < a.M; ++m) {
                 const uint32_t g = a.sweep_base + (uint32_t)(s * a.M + m);
-                for (int tau = 0; tau < 2; ++tau) {
-                    const int tsa = tau == 0 ? 1 : W - 1, tsb = tau == 0 ? W - 1 : 1;  // rotation by one slice
-                    // the same rotation as one funnel shift of (O : O << rls) right by rsh
-                    const int rls = (W == 1 && tau == 1) ? 0 : 32 - W, rsh = (32 - tsa) & 31;
+                // The cached sweep runs all of it in one pass of this loop (three
+                // phases, below); the general one a pass per Trotter parity.
+                for (int tau = 0; tau < (CACHED ? 1 : 2); ++tau) {
                     // The top cnt queued words, one per lane: each word's stochastic
                     // bits meet the same bit-sliced uniforms, thresholds and early
                     // exit as in the inline path, and the accepted ones are XORed
@@ -368,7 +367,8 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                         const int lane = tid & 63;
                         if (lane < cnt) {
                             const uint4 rec = wq2[q2 - cnt + lane];
-                            const int i = (int)(rec.x & 0xffffu), rr = (int)(rec.x >> 16);
+                            const int i = (int)(rec.x & 0xffffu), rr = (int)((rec.x >> 16) & 0x7fffu);
+                            const int tau = (int)(rec.x >> 31);  // the word's Trotter parity
                             const WT m0 = (WT)rec.y;
                             WT lt = (WT)rec.z, und = (WT)rec.w;
                             const uint32_t rep = a.replica_base + (uint32_t)(r0 + rr);
@@ -399,7 +399,8 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                         uint4 rec2 = make_uint4(0u, 0u, 0u, 0u);
                         if (lane < cnt) {
                             const uint2 r2 = wq1[qn - cnt + lane];
-                            const int i = (int)(r2.x & 0xffffu), rr = (int)(r2.x >> 16);
+                            const int i = (int)(r2.x & 0xffffu), rr = (int)((r2.x >> 16) & 0x7fffu);
+                            const int tau = (int)(r2.x >> 31);
                             const WT m0 = (WT)r2.y;
                             WT lt = 0, und = m0;
                             uint32_t out[4];
@@ -439,12 +440,13 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                         __builtin_amdgcn_wave_barrier();
                         const int lane = tid & 63;
                         if (lane < qn + q2) {
-                            int i, rr, qd0;
+                            int i, rr, qd0, tau;
                             WT m0, lt, und;
                             if (lane < qn) {
                                 const uint2 r2 = wq1[lane];
                                 i = (int)(r2.x & 0xffffu);
-                                rr = (int)(r2.x >> 16);
+                                rr = (int)((r2.x >> 16) & 0x7fffu);
+                                tau = (int)(r2.x >> 31);
                                 m0 = (WT)r2.y;
                                 lt = 0;
                                 und = m0;
@@ -452,7 +454,8 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                             } else {
                                 const uint4 rec = wq2[lane - qn];
                                 i = (int)(rec.x & 0xffffu);
-                                rr = (int)(rec.x >> 16);
+                                rr = (int)((rec.x >> 16) & 0x7fffu);
+                                tau = (int)(rec.x >> 31);
                                 m0 = (WT)rec.y;
                                 lt = (WT)rec.z;
                                 und = (WT)rec.w;
@@ -488,7 +491,8 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                             } else {
                                 rec = wq[qn - cnt + lane];
                             }
-                            const int i = (int)(rec.x & 0xffffu), rr = (int)(rec.x >> 16);
+                            const int i = (int)(rec.x & 0xffffu), rr = (int)((rec.x >> 16) & 0x7fffu);
+                            const int tau = (int)(rec.x >> 31);
                             const WT m0 = (WT)rec.y, m1 = NS() > 1 ? (WT)rec.z : WT(0), m2 = NS() > 2 ? (WT)rec.w : WT(0);
                             const uint32_t rep = a.replica_base + (uint32_t)(r0 + rr);
                             WT lt = 0, und = m0 | m1 | m2;
@@ -515,7 +519,10 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                     // for every replica of the workgroup; lanes with act false take
                     // part only in the queue's wave-wide steps
                     auto spin = [&](const bool act, const int q, const int i, const uint32_t info, const int zi,
-                                    const int *nb, const FieldClasses *fcp) __attribute__((always_inline)) {
+                                    const int *nb, const FieldClasses *fcp, const int tau) __attribute__((always_inline)) {
+                            const int tsa = tau == 0 ? 1 : W - 1, tsb = tau == 0 ? W - 1 : 1;  // rotation by one slice
+                            // the same rotation as one funnel shift of (O : O << rls) right by rsh
+                            const int rls = (W == 1 && tau == 1) ? 0 : 32 - W, rsh = (32 - tsa) & 31;
                             const uint32_t zt = UZ ? zt_u : ((PMJ && !FLD) ? tz[zi] : 0u);
                             // fields: this spin's class thresholds, shared by the replicas below
                             // (cached spins: computed once per field step)
@@ -821,9 +828,9 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                                                    (uint32_t)(bal >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
                                           if (SINGLE)
-                                              wq1[pos] = make_uint2((uint32_t)i | ((uint32_t)rr << 16), (uint32_t)qm0);
+                                              wq1[pos] = make_uint2((uint32_t)i | ((uint32_t)rr << 16) | ((uint32_t)tau << 31), (uint32_t)qm0);
                                           else
-                                              wq[pos] = make_uint4((uint32_t)i | ((uint32_t)rr << 16), (uint32_t)qm0,
+                                              wq[pos] = make_uint4((uint32_t)i | ((uint32_t)rr << 16) | ((uint32_t)tau << 31), (uint32_t)qm0,
                                                                    (uint32_t)qm1, (uint32_t)qm2);
                                       }
                                       qn += (int)__popcll(bal);
@@ -836,8 +843,21 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                             }
                     };
                     if constexpr (CACHED) {
+                        // Three phases: (parity 0, color 0); (parity 0, color 1)
+                        // together with (parity 1, color 0); (parity 1, color 1).
+                        // The two updates of the middle phase touch disjoint words
+                        // and neither reads what the other writes (parity 0 of
+                        // color 1 reads parity 0 of color 0 and parity 1 of color
+                        // 1; parity 1 of color 0 reads parity 1 of color 1 and
+                        // parity 0 of color 0), so running them in one phase gives
+                        // the sequential order's spins, with one partial queue batch
+                        // and one barrier fewer per sweep.
     #pragma unroll
-                        for (int c = 0; c < 2; ++c) {
+                        for (int ph = 0; ph < 3; ++ph) {
+    #pragma unroll
+                          for (int part = 0; part < (ph == 1 ? 2 : 1); ++part) {
+                            const int tp = ph == 0 ? 0 : (ph == 2 ? 1 : part);
+                            const int c = ph == 0 ? 0 : (ph == 2 ? 1 : 1 - part);
     #pragma unroll
                             for (int k = 0; k < 2; ++k) {
                                 // (c, k) are compile-time after unrolling: the cached registers directly
@@ -852,8 +872,9 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                     nb[z] = c0k ? (k0k ? cN[0][0][z] : cN[0][1][z]) : (k0k ? cN[1][0][z] : cN[1][1][z]);
                                 const FieldClasses *fcp = nullptr;
                                 if constexpr (FLD) fcp = c0k ? (k0k ? &cfc[0][0] : &cfc[0][1]) : (k0k ? &cfc[1][0] : &cfc[1][1]);
-                                spin(q >= 0, q, i, info, ZU, nb, fcp);
+                                spin(q >= 0, q, i, info, ZU, nb, fcp, tp);
                             }
+                          }
                             if constexpr (CQ && SINGLE) {
                                 // full batches first, then one mixed pass over the rest
                                 while (qn + q2 > 64) {
@@ -890,7 +911,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                             if (z < a.zmax) nb[z] = z < zi ? a.cnb[(size_t)z * N + q] : i;
                                     }
                                 }
-                                spin(act, q, i, info, zi, nb, nullptr);
+                                spin(act, q, i, info, zi, nb, nullptr, tau);
                             }
                             if constexpr (CQ && SINGLE) {
                                 // full batches first, then one mixed pass over the rest
