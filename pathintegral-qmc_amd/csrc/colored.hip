@@ -46,19 +46,24 @@ constexpr int kScratchMaxDevices = 64;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 unsigned char *g_scratch_ptr[kScratchMaxDevices] = {};
 std::mutex g_scratch_mu;
-// Stochastic-word queues of the +-1 sweep, per wavefront, never holding more
-// than 63 + 64 records between drains: with several classes one queue of
-// 16-byte records (index | replica << 16, three class masks); with a single
-// class a queue of 8-byte records (index | replica << 16, mask) and a second
-// one of 16-byte records for words still undecided after their first Philox
-// block (index | replica << 16, mask, accepted bits, undecided bits).
-constexpr int kQueueCap = 128;
-constexpr int kQueueBytes = 1024 + 2048;  // per wavefront: [128] uint2 + [128] uint4, or [128] uint4
+// Stochastic-word queues of the +-1 sweep, per wavefront: with several
+// classes one queue of 16-byte records (index | replica << 16 | parity << 31,
+// three class masks); with a single class a queue of 8-byte records (index |
+// replica << 16 | parity << 31, mask) and a second one of 16-byte records for
+// words still undecided after their first Philox block (the same, then
+// accepted bits, undecided bits).  Fresh records are drained 64 at a time once
+// 64 are queued, after every replica's append (at most 63 + 64 queued) or, in
+// the fixed-plane sweep, after both replicas' (at most 63 + 2 x 64); the
+// second queue holds at most 63 + 64.
+constexpr int kQueueCap1 = 192, kQueueCap2 = 128;
+constexpr int kQueueBytes = 8 * kQueueCap1 + 16 * kQueueCap2;  // per wavefront
+// Geometry of the fixed-plane cached sweep (see colored_body's FIXED).
+constexpr int kFixRep = 2, kFixPlane = 1024;
 
 // LDS of the word-per-thread sweep: packed state [rpw][2][N] words, class
 // thresholds [kClassStride], per-degree table [8], reduction scratch [16]
 // doubles, then (16-byte aligned) the wavefronts' stochastic queues.
-__host__ __device__ inline size_t queue_offset(int N, int rpw, int word_bytes)
+__host__ __device__ constexpr inline size_t queue_offset(int N, int rpw, int word_bytes)
 {
     return ((size_t)word_bytes * 2 * (size_t)N * rpw + 4 * (kClassStride + 8) + 16 * sizeof(double) + 15) &
            ~(size_t)15;
@@ -248,25 +253,40 @@ __device__ inline uint32_t rot_slices<uint32_t>(uint32_t O, int, int, int rls, i
     return __builtin_amdgcn_alignbit(O, O << rls, (uint32_t)rsh);
 }
 
-template <typename WT, bool PMJ, bool UZ = false, bool FLD = false, int ZU = 0, bool CACHED = false, bool CQ = false>
-__device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned char *smem)
+// FIXED (cached +-1 queue sweep only): exactly kFixRep replicas per workgroup,
+// each (replica, parity) plane kFixPlane words apart whatever N (N <= kFixPlane),
+// and every thread holding a spin of each color in both of its slots.  All
+// plane offsets are then compile-time, so every LDS access of the word body
+// takes its plane as the instruction's immediate offset, and no lane is
+// masked off.
+template <typename WT, bool PMJ, bool UZ = false, bool FLD = false, int ZU = 0, bool CACHED = false, bool CQ = false,
+          bool FIXED = false>
+__device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned char *smem)
 {
     static_assert(!CQ || (UZ && PMJ && !FLD && sizeof(WT) == 4), "stochastic queue: +-1 uniform-degree 32-bit path");
+    static_assert(!FIXED || (CACHED && CQ), "fixed planes: cached queue sweep");
     const int N = a.N, W = a.W, tid = threadIdx.x, bd = blockDim.x;
-    WT *st = reinterpret_cast<WT *>(smem);                            // [nrep][2][N]
-    uint32_t *tthr = reinterpret_cast<uint32_t *>(st + (size_t)2 * N * a.rpw);  // [kClassStride]
+    const int nrep = FIXED ? kFixRep : nrep_;
+    const int PW = FIXED ? kFixPlane : N;                             // words per (replica, parity) plane
+    const int rpw = FIXED ? kFixRep : a.rpw;
+    WT *st = reinterpret_cast<WT *>(smem);                            // [nrep][2][PW]
+    uint32_t *tthr = reinterpret_cast<uint32_t *>(st + (size_t)2 * PW * rpw);  // [kClassStride]
     uint32_t *tz = tthr + kClassStride;                               // [8]
     double *red = reinterpret_cast<double *>(tz + 8);                 // [16]
     // per-wavefront queues of stochastic words (CQ), kQueueBytes each
-    unsigned char *wqb = smem + queue_offset(a.N, a.rpw, sizeof(WT)) + (size_t)(tid >> 6) * kQueueBytes;
+    unsigned char *wqb = smem + queue_offset(PW, rpw, sizeof(WT)) + (size_t)(tid >> 6) * kQueueBytes;
     uint4 *wq = reinterpret_cast<uint4 *>(wqb);           // several classes
     uint2 *wq1 = reinterpret_cast<uint2 *>(wqb);          // single class: fresh words
-    uint4 *wq2 = reinterpret_cast<uint4 *>(wqb + 1024);   // single class: undecided after block 0
+    uint4 *wq2 = reinterpret_cast<uint4 *>(wqb + 8 * kQueueCap1);  // single class: undecided after block 0
     int qn = 0;  // queued records of this wavefront (wave-uniform)
     int q2 = 0;  // records in wq2 (wave-uniform)
     const WT valid = (W == (int)(8 * sizeof(WT))) ? ~WT(0) : ((WT(1) << W) - 1);
     WT *gst = a.packed + (size_t)r0 * 2 * N;
-    for (int q = tid; q < 2 * N * nrep; q += bd) st[q] = gst[q];
+    if constexpr (FIXED) {
+        for (int q = tid; q < 2 * N * nrep; q += bd) st[(q / N) * PW + q % N] = gst[q];
+    } else {
+        for (int q = tid; q < 2 * N * nrep; q += bd) st[q] = gst[q];
+    }
     constexpr int ZC = ZU > 0 ? ZU : 1;
     int cq_[2][2], cI[2][2], cN[2][2][ZC];
     uint32_t cF[2][2];
@@ -383,7 +403,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                     und &= ~(Rb ^ tb);
                                 }
                             }
-                            st[((size_t)rr * 2 + tau) * N + i] ^= lt;
+                            st[((size_t)rr * 2 + tau) * PW + i] ^= lt;
                         }
                         q2 -= cnt;
                         __builtin_amdgcn_wave_barrier();
@@ -414,7 +434,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 und &= ~(Rb ^ tb);
                             }
                             if (und == 0) {
-                                st[((size_t)rr * 2 + tau) * N + i] ^= lt;
+                                st[((size_t)rr * 2 + tau) * PW + i] ^= lt;
                             } else {
                                 more = true;
                                 rec2 = make_uint4(r2.x, r2.y, (uint32_t)lt, (uint32_t)und);
@@ -473,7 +493,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                     und &= ~(Rb ^ tb);
                                 }
                             }
-                            st[((size_t)rr * 2 + tau) * N + i] ^= lt;
+                            st[((size_t)rr * 2 + tau) * PW + i] ^= lt;
                         }
                         qn = 0;
                         q2 = 0;
@@ -510,7 +530,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                     und &= ~(Rb ^ tb);
                                 }
                             }
-                            st[((size_t)rr * 2 + tau) * N + i] ^= lt;
+                            st[((size_t)rr * 2 + tau) * PW + i] ^= lt;
                         }
                         qn -= cnt;
                         __builtin_amdgcn_wave_barrier();
@@ -533,11 +553,13 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 if (fcp) fc = *fcp;
                                 else field_classes(hfield, zi, jp, Ts, fc);
                             }
+                            // fixed planes: unrolled, so each replica's planes are immediates
+    #pragma unroll(FIXED ? kFixRep : 1)
                             for (int rr = 0; rr < nrep; ++rr) {
                               WT qX = 0, qm0 = 0, qm1 = 0, qm2 = 0;  // stochastic word for the queue
                               if (act) {
-                                WT *mine = st + ((size_t)rr * 2 + tau) * N;
-                                const WT *other = st + ((size_t)rr * 2 + 1 - tau) * N;
+                                WT *mine = st + ((size_t)rr * 2 + tau) * PW;
+                                const WT *other = st + ((size_t)rr * 2 + 1 - tau) * PW;
                                 const uint32_t rep = a.replica_base + (uint32_t)(r0 + rr);
                                 const WT S_ = mine[i], O = other[i];
                                 // Trotter neighbours of slice 2m+tau: bits m of O and of O
@@ -834,7 +856,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                                                    (uint32_t)qm1, (uint32_t)qm2);
                                       }
                                       qn += (int)__popcll(bal);
-                                      if (qn >= 64) {
+                                      if (!FIXED && qn >= 64) {
                                           if constexpr (SINGLE) drain1(64);
                                           else drain(64, std::integral_constant<int, 3>());
                                       }
@@ -852,29 +874,41 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                         // parity 0 of color 0), so running them in one phase gives
                         // the sequential order's spins, with one partial queue batch
                         // and one barrier fewer per sweep.
+                        // Each slot (phase, part, k) is its own instantiation, so
+                        // (parity, color, k) are compile-time whatever the size of
+                        // the code and the cached registers are used directly.
+                        auto slot = [&](auto PH, auto PART, auto K) __attribute__((always_inline)) {
+                            constexpr int ph = decltype(PH)::value, part = decltype(PART)::value;
+                            constexpr int k = decltype(K)::value;
+                            constexpr int tp = ph == 0 ? 0 : (ph == 2 ? 1 : part);
+                            constexpr int c = ph == 0 ? 0 : (ph == 2 ? 1 : 1 - part);
+                            const int q = cq_[c][k];
+                            if (!CQ && q < 0) return;
+                            int nb[ZC];
     #pragma unroll
-                        for (int ph = 0; ph < 3; ++ph) {
-    #pragma unroll
-                          for (int part = 0; part < (ph == 1 ? 2 : 1); ++part) {
-                            const int tp = ph == 0 ? 0 : (ph == 2 ? 1 : part);
-                            const int c = ph == 0 ? 0 : (ph == 2 ? 1 : 1 - part);
-    #pragma unroll
-                            for (int k = 0; k < 2; ++k) {
-                                // (c, k) are compile-time after unrolling: the cached registers directly
-                                const bool c0k = c == 0, k0k = k == 0;
-                                const int q = c0k ? (k0k ? cq_[0][0] : cq_[0][1]) : (k0k ? cq_[1][0] : cq_[1][1]);
-                                if (!CQ && q < 0) continue;
-                                const int i = c0k ? (k0k ? cI[0][0] : cI[0][1]) : (k0k ? cI[1][0] : cI[1][1]);
-                                const uint32_t info = c0k ? (k0k ? cF[0][0] : cF[0][1]) : (k0k ? cF[1][0] : cF[1][1]);
-                                int nb[ZC];
-    #pragma unroll
-                                for (int z = 0; z < ZC; ++z)
-                                    nb[z] = c0k ? (k0k ? cN[0][0][z] : cN[0][1][z]) : (k0k ? cN[1][0][z] : cN[1][1][z]);
-                                const FieldClasses *fcp = nullptr;
-                                if constexpr (FLD) fcp = c0k ? (k0k ? &cfc[0][0] : &cfc[0][1]) : (k0k ? &cfc[1][0] : &cfc[1][1]);
-                                spin(q >= 0, q, i, info, ZU, nb, fcp, tp);
+                            for (int z = 0; z < ZC; ++z) nb[z] = cN[c][k][z];
+                            const FieldClasses *fcp = nullptr;
+                            if constexpr (FLD) fcp = &cfc[c][k];
+                            spin(FIXED || q >= 0, q, cI[c][k], cF[c][k], ZU, nb, fcp, tp);
+                            if constexpr (FIXED && !DET) {
+                                // one drain site for both replicas' appends
+                                if (cqm) {
+                                    while (qn >= 64) {
+                                        if constexpr (SINGLE) drain1(64);
+                                        else drain(64, std::integral_constant<int, 3>());
+                                    }
+                                }
                             }
-                          }
+                        };
+                        auto phase = [&](auto PH) __attribute__((always_inline)) {
+                            using I0 = std::integral_constant<int, 0>;
+                            using I1 = std::integral_constant<int, 1>;
+                            slot(PH, I0(), I0());
+                            slot(PH, I0(), I1());
+                            if constexpr (decltype(PH)::value == 1) {
+                                slot(PH, I1(), I0());
+                                slot(PH, I1(), I1());
+                            }
                             if constexpr (CQ && SINGLE) {
                                 // full batches first, then one mixed pass over the rest
                                 while (qn + q2 > 64) {
@@ -886,7 +920,10 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
                                 drain(qn, std::integral_constant<int, 3>());
                             }
                             __syncthreads();
-                        }
+                        };
+                        phase(std::integral_constant<int, 0>());
+                        phase(std::integral_constant<int, 1>());
+                        phase(std::integral_constant<int, 2>());
                     } else {
                         for (int c = 0; c < a.ncolors; ++c) {
                             const int q0 = a.color_off[c], q1 = a.color_off[c + 1];
@@ -942,7 +979,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
             // Mean over slices of the classical energy; sums of +-1 products are
             // popcounts of XORed parity words.
             for (int rr = 0; rr < nrep; ++rr) {
-                const WT *s0 = st + (size_t)rr * 2 * N, *s1 = s0 + N;
+                const WT *s0 = st + (size_t)rr * 2 * PW, *s1 = s0 + PW;
                 double acc = 0.0;
                 for (int i = tid; i < N; i += bd) {
                     const WT a0 = s0[i], a1 = s1[i];
@@ -966,7 +1003,11 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep, unsigned ch
         }
     }
     __syncthreads();
-    for (int q = tid; q < 2 * N * nrep; q += bd) gst[q] = st[q];
+    if constexpr (FIXED) {
+        for (int q = tid; q < 2 * N * nrep; q += bd) gst[q] = st[(q / N) * PW + q % N];
+    } else {
+        for (int q = tid; q < 2 * N * nrep; q += bd) gst[q] = st[q];
+    }
 }
 
 // flatten: the sweep body is inlined whole (its unrolled phases need the
@@ -1001,14 +1042,37 @@ __global__ void __attribute__((flatten)) __attribute__((amdgpu_waves_per_eu(5)))
     colored_body<WT, true, true, false, ZU, CACHED, true>(a, r0, min(a.rpw, a.R - r0), smem);
 }
 
+// The fixed-plane cached queue sweep (square lattices, 32-bit words): 256
+// threads, two replicas, N <= kFixPlane.  Its LDS is a static array, so the
+// plane offsets and the array's base are all compile-time and fold into the
+// LDS instructions' immediate offsets.  Four wavefronts per SIMD: with the
+// replica loop unrolled the body needs about 128 registers, and the budget of
+// five (96 registers) spills in the word loop (1024-spin workload: 12.3 ms
+// against 9.25 ms).
+constexpr size_t kFixBlock = 256;
+constexpr size_t kFixLds = queue_offset(kFixPlane, kFixRep, 4) + kFixBlock / 64 * kQueueBytes;
+__global__ void __attribute__((flatten)) __attribute__((amdgpu_waves_per_eu(4))) __launch_bounds__(kFixBlock)
+colored_q_fixed_kernel(ColArgs<uint32_t> a)
+{
+    __shared__ __align__(16) unsigned char fsm[kFixLds];
+    colored_body<uint32_t, true, true, false, 4, true, true, true>(a, blockIdx.x * kFixRep, kFixRep, fsm);
+}
+
 // Launch of the +-1 sweep for a uniform-degree lattice of compile-time degree
 // ZU, with per-thread cached neighbour lists when two colors of at most two
 // spins per thread allow it, and with the per-wavefront stochastic queue when
 // its LDS fits (32-bit words).
 template <typename WT, int ZU, bool CQ>
-void launch_uniform_q(const ColArgs<WT> &a, bool cached, int grid, int block, size_t lds, hipStream_t st)
+void launch_uniform_q(const ColArgs<WT> &a, bool cached, int grid, int block, size_t lds, hipStream_t st,
+                      bool fixed = false)
 {
     if constexpr (CQ) {
+        if constexpr (ZU == 4 && sizeof(WT) == 4) {
+            if (fixed) {
+                colored_q_fixed_kernel<<<grid, (int)kFixBlock, 0, st>>>(a);
+                return;
+            }
+        }
         if (cached) {
             (void)hipFuncSetAttribute((const void *)colored_q_kernel<WT, ZU, true>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1053,6 +1117,24 @@ inline bool cached_fields_enabled()
     return !(e && e[0] == '0');
 }
 
+// The fixed-plane cached sweep runs where it applies unless PIQMC_FIXED_PLANES=0
+// (identical results; the switch exists to compare).
+inline bool fixed_planes_enabled()
+{
+    const char *e = std::getenv("PIQMC_FIXED_PLANES");
+    return !(e && e[0] == '0');
+}
+
+// The fixed-plane kernel applies to +-1 problems of uniform degree 4 with 32-bit
+// words, two colors of exactly 2 x kFixBlock spins, N <= kFixPlane and an even
+// replica count (at kFixRep replicas per workgroup and kFixBlock threads).
+inline bool fixed_tile_applies(const piqmc_problem *pb, int R, size_t word_bytes)
+{
+    return fixed_planes_enabled() && queue_enabled() && word_bytes == 4 && pb->pmj && pb->zuni == 4 &&
+           pb->ncolors == 2 && pb->N <= kFixPlane && R % kFixRep == 0 &&
+           pb->color_off[1] == 2 * (int)kFixBlock && pb->color_off[2] - pb->color_off[1] == 2 * (int)kFixBlock;
+}
+
 // LDS of the word-per-thread sweep with the stochastic queues of block/64 wavefronts.
 inline size_t queue_lds(int N, int rpw, int word_bytes, int block)
 {
@@ -1067,6 +1149,14 @@ void launch_uniform(const ColArgs<WT> &a, const piqmc_problem *pb, int grid, int
     const bool cached = pb->ncolors == 2 && biggest <= 2 * block;
     const size_t lq = queue_lds(a.N, a.rpw, sizeof(WT), block);
     if constexpr (sizeof(WT) == 4) {
+        // fixed planes: two replicas per workgroup (R even), both colors
+        // exactly two spins per thread, N within a plane
+        const bool fixed = ZU == 4 && a.rpw == kFixRep && block == (int)kFixBlock &&
+                           fixed_tile_applies(pb, a.R, sizeof(WT));
+        if (fixed && queue_enabled()) {
+            launch_uniform_q<WT, ZU, true>(a, cached, grid, block, 0, st, true);
+            return;
+        }
         if (lq <= 160 * 1024 && queue_enabled()) {
             launch_uniform_q<WT, ZU, true>(a, cached, grid, block, lq, st);
             return;
@@ -2448,6 +2538,12 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
     }
     std::vector<uint32_t> thr, ztab;
     const bool pmj = pb->pmj && pmj_tables(pb, S, jp, temps, thr, ztab);
+    if (rpw <= 0 && block <= 0 && fixed_tile_applies(pb, R, sizeof(WT))) {
+        // the fixed-plane kernel's tile (1024-spin workload: 9.25 ms against
+        // 9.6 ms for the best general tile)
+        rpw = kFixRep;
+        block = (int)kFixBlock;
+    }
     if (rpw <= 0) rpw = R >= 2048 ? 4 : (R >= 1024 ? 2 : 1);
     if (rpw > 64) return piqmc::set_error(PIQMC_EINVAL, "replicas per workgroup must be <= 64");
     const size_t lds = sizeof(WT) * 2 * (size_t)N * rpw + 4 * (kClassStride + 8) + 16 * sizeof(double);
