@@ -311,7 +311,11 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
         if constexpr (CACHED) return *reinterpret_cast<const WT *>(reinterpret_cast<const char *>(base) + nb);
         else return base[nb];
     };
-    constexpr int KMAX = 8;  // thresholds clamp at 8 ("count >= 8" is empty)
+    // thresholds clamp at 8 ("count >= 8" is empty).  Nine masks also keep the
+    // runtime-indexed selections below as indexed register moves: with six
+    // (degree 4 needs no more) the compiler expands each into a chain of
+    // selects, and the 1024-spin sweep runs 9.6 -> 12.5 ms.
+    constexpr int KMAX = 8;
 
     for (int s = 0; s < a.S; ++s) {
         __syncthreads();
@@ -640,15 +644,11 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                                     } else if (UZ) {
                                         // thresholds are wave-uniform: every "count >= k" mask
                                         // once, then six selections by a uniform index
-                                        ge[0] = ~WT(0);
-                                        ge[1] = c0 | c1 | c2;
-                                        ge[2] = c1 | c2;
-                                        ge[3] = c2 | (c1 & c0);
-                                        ge[4] = c2;
-                                        ge[5] = c2 & (c1 | c0);
-                                        ge[6] = c2 & c1;
-                                        ge[7] = c2 & c1 & c0;
-                                        ge[8] = WT(0);
+                                        const WT gek[9] = {~WT(0), c0 | c1 | c2, c1 | c2, c2 | (c1 & c0), c2,
+                                                           c2 & (c1 | c0), c2 & c1, c2 & c1 & c0, WT(0)};
+    #pragma unroll
+                                        for (int kk = 0; kk < KMAX; ++kk) ge[kk] = gek[kk];
+                                        ge[KMAX] = WT(0);
                                         flips = (e0 & ge[ka0]) | (e1 & ge[ka1]) | (e2 & ge[ka2]);
                                         if constexpr (SINGLE) {
                                             // the step's one stochastic class (u_t, u_s): exactly the
@@ -849,11 +849,15 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                                           const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi(
                                                                    (uint32_t)(bal >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                                          // formed here: hoisted out of the step loop, the
+                                          // records' fixed parts would hold registers
+                                          uint32_t rx = (uint32_t)i;
+                                          asm volatile("" : "+v"(rx));
+                                          rx |= ((uint32_t)rr << 16) | ((uint32_t)tau << 31);
                                           if (SINGLE)
-                                              wq1[pos] = make_uint2((uint32_t)i | ((uint32_t)rr << 16) | ((uint32_t)tau << 31), (uint32_t)qm0);
+                                              wq1[pos] = make_uint2(rx, (uint32_t)qm0);
                                           else
-                                              wq[pos] = make_uint4((uint32_t)i | ((uint32_t)rr << 16) | ((uint32_t)tau << 31), (uint32_t)qm0,
-                                                                   (uint32_t)qm1, (uint32_t)qm2);
+                                              wq[pos] = make_uint4(rx, (uint32_t)qm0, (uint32_t)qm1, (uint32_t)qm2);
                                       }
                                       qn += (int)__popcll(bal);
                                       if (!FIXED && qn >= 64) {
@@ -1045,13 +1049,13 @@ __global__ void __attribute__((flatten)) __attribute__((amdgpu_waves_per_eu(5)))
 // The fixed-plane cached queue sweep (square lattices, 32-bit words): 256
 // threads, two replicas, N <= kFixPlane.  Its LDS is a static array, so the
 // plane offsets and the array's base are all compile-time and fold into the
-// LDS instructions' immediate offsets.  Four wavefronts per SIMD: with the
-// replica loop unrolled the body needs about 128 registers, and the budget of
-// five (96 registers) spills in the word loop (1024-spin workload: 12.3 ms
-// against 9.25 ms).
+// LDS instructions' immediate offsets.  Five wavefronts per SIMD (96
+// registers): the word loops fit without spilling, the few spills left are in
+// the per-phase queue tails (1024-spin workload: 8.91 ms, against 9.04 ms at
+// four wavefronts and 106 registers).
 constexpr size_t kFixBlock = 256;
 constexpr size_t kFixLds = queue_offset(kFixPlane, kFixRep, 4) + kFixBlock / 64 * kQueueBytes;
-__global__ void __attribute__((flatten)) __attribute__((amdgpu_waves_per_eu(4))) __launch_bounds__(kFixBlock)
+__global__ void __attribute__((flatten)) __attribute__((amdgpu_waves_per_eu(5))) __launch_bounds__(kFixBlock)
 colored_q_fixed_kernel(ColArgs<uint32_t> a)
 {
     __shared__ __align__(16) unsigned char fsm[kFixLds];
