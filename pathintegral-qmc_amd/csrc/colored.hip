@@ -747,6 +747,9 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                                         if (FLD && ns > 3) {
                                             // many stochastic triples (high temperature): each
                                             // stochastic bit assembles its own uniform and threshold
+                                            // (out of line as a called function it costs more in the
+                                            // call's register traffic than it saves: 297 -> 301 ms
+                                            // on the 3-D L=12 field workload)
                                             for (int mm = 0; mm < W; ++mm) {
                                                 if (!((X >> mm) & WT(1))) continue;
                                                 const int sg = (int)((S_ >> mm) & WT(1));
@@ -778,7 +781,8 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                                                 const WT Rb = rword<WT>(out, t);
                                                 const int sh = 31 - (qd * kPer + t);
                                                 WT tb;  // bits whose class threshold has bit sh set
-                                                if (ns <= 3) {
+                                                // (with fields more than three classes never get here)
+                                                if (FLD || ns <= 3) {
                                                     tb = (((ct0 >> sh) & 1u) ? cm0 : WT(0)) | (((ct1 >> sh) & 1u) ? cm1 : WT(0)) |
                                                          (((ct2 >> sh) & 1u) ? cm2 : WT(0));
                                                 } else {
