@@ -1291,9 +1291,10 @@ __device__ void dense_replica(const ColArgs<WT> &a, int r, unsigned char *base)
             for (int tau = 0; tau < 2; ++tau) {
                 WT *mine = st + tau * N;
                 const WT *other = st + (1 - tau) * N;
-                fetch_row(__builtin_amdgcn_readfirstlane(a.order[0]));
+                int inext = __builtin_amdgcn_readfirstlane(a.order[0]);
+                fetch_row(inext);
                 for (int q = 0; q < N; ++q) {  // colors in order; spins of one color are independent
-                    const int i = __builtin_amdgcn_readfirstlane(a.order[q]);  // wave-uniform
+                    const int i = inext;  // wave-uniform
                     // stage the row: couplings and the current words of the neighbours
                     // (a field entry stages word 0, so its term keeps the sign of J)
 #pragma unroll
@@ -1312,7 +1313,11 @@ __device__ void dense_replica(const ColArgs<WT> &a, int r, unsigned char *base)
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    if (q + 1 < N) fetch_row(__builtin_amdgcn_readfirstlane(a.order[q + 1]));
+                    // the next index, read once for its row prefetch and carried over
+                    if (q + 1 < N) {
+                        inext = __builtin_amdgcn_readfirstlane(a.order[q + 1]);
+                        fetch_row(inext);
+                    }
                     const WT S_ = mine[i], O = other[i];
                     const int lb = parts > 1 ? lane % W : (lane < W ? lane : 0);
                     // Trotter neighbours of slice 2lb+tau: slices 2lb+tau-1 and 2lb+tau+1
@@ -1503,9 +1508,10 @@ __device__ void dense_wg_replica(const ColArgs<WT> &a, int r, unsigned char *bas
             for (int tau = 0; tau < 2; ++tau) {
                 WT *mine = st + tau * N;
                 const WT *other = st + (1 - tau) * N;
-                fetch_row(__builtin_amdgcn_readfirstlane(a.order[0]));
+                int inext = __builtin_amdgcn_readfirstlane(a.order[0]);
+                fetch_row(inext);
                 for (int q = 0; q < N; ++q) {
-                    const int i = __builtin_amdgcn_readfirstlane(a.order[q]);
+                    const int i = inext;
 #pragma unroll
                     for (int k = 0; k < kPre; ++k) {
                         const int e = tid + 256 * k;
@@ -1520,7 +1526,11 @@ __device__ void dense_wg_replica(const ColArgs<WT> &a, int r, unsigned char *bas
                         wb[e] = rw.x == i ? WT(0) : mine[rw.x];
                     }
                     __syncthreads();
-                    if (q + 1 < N) fetch_row(__builtin_amdgcn_readfirstlane(a.order[q + 1]));
+                    // the next index, read once for its row prefetch and carried over
+                    if (q + 1 < N) {
+                        inext = __builtin_amdgcn_readfirstlane(a.order[q + 1]);
+                        fetch_row(inext);
+                    }
                     // this lane's entries gp, gp + stride, ...: with one lane per
                     // slice (long runs) in groups of G read one group ahead, four
                     // chains; with several lanes per slice (runs of 8-32) two
@@ -1986,15 +1996,18 @@ __device__ void dense_fields_replica(const ColArgs<WT> &a, int r, unsigned char 
                 const WT *other = st + (1 - tau) * N;
                 double *gtt = gt + (size_t)tau * W * NP;
                 for (int t = lane; t < W * NP; t += 64) tab[t] = gtt[t];
-                fetch_row(__builtin_amdgcn_readfirstlane(ord[0]));
+                int inext = __builtin_amdgcn_readfirstlane(ord[0]);
+                fetch_row(inext);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 for (int q = 0; q < N; ++q) {
-                    const int i = __builtin_amdgcn_readfirstlane(ord[q]);
+                    const int i = inext;
 #pragma unroll
                     for (int k = 0; k < kPre; ++k) cur[k] = pre[k];
-                    if (q + 1 < N) fetch_row(__builtin_amdgcn_readfirstlane(ord[q + 1]));
+                    // the next index, read once for its row prefetch and carried over
+                    // (re-reading it put one more LDS round trip on the decision chain)
+                    if (q + 1 < N) { inext = __builtin_amdgcn_readfirstlane(ord[q + 1]); fetch_row(inext); }
                     const double h = tab[(size_t)lb * NP + i];
                     const WT S_ = mine[i], O = other[i];
                     const WT bal = dense_decide<WT, true>(a, h, S_, O, lane, tau, jp, Ts, g, i, rep);
@@ -2111,15 +2124,18 @@ __device__ void dense_fields_split(const ColArgs<WT> &a, int r, unsigned char *b
                 const WT *other = st + (1 - tau) * N;
                 double *gtt = gt + (size_t)tau * W * NP + (size_t)s0 * NP;
                 for (int t = lane; t < Ws * NP; t += 64) tab[t] = gtt[t];
-                fetch_row(__builtin_amdgcn_readfirstlane(ord[0]));
+                int inext = __builtin_amdgcn_readfirstlane(ord[0]);
+                fetch_row(inext);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 for (int q = 0; q < N; ++q) {
-                    const int i = __builtin_amdgcn_readfirstlane(ord[q]);
+                    const int i = inext;
 #pragma unroll
                     for (int k = 0; k < kPre; ++k) cur[k] = pre[k];
-                    if (q + 1 < N) fetch_row(__builtin_amdgcn_readfirstlane(ord[q + 1]));
+                    // the next index, read once for its row prefetch and carried over
+                    // (re-reading it put one more LDS round trip on the decision chain)
+                    if (q + 1 < N) { inext = __builtin_amdgcn_readfirstlane(ord[q + 1]); fetch_row(inext); }
                     const double h = tab[(size_t)ll * NP + i];
                     const WT S_ = mine[i], O = other[i];
                     // the ballot is over lanes: lane l decided slice s0 + l
