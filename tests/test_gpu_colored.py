@@ -353,15 +353,16 @@ def test_colored_stochastic_queue(gpu, monkeypatch, queue, L, dim, P, T, S, bloc
 
 
 @pytest.mark.parametrize("fixed", ["1", "0"])
-@pytest.mark.parametrize("P,T,S", [(64, 1 / 64, 30), (20, 0.05, 25), (2, 0.5, 15)])
-def test_colored_fixed_planes(gpu, monkeypatch, fixed, P, T, S):
+@pytest.mark.parametrize("P,T,S,M", [(64, 1 / 64, 30, 1), (20, 0.05, 25, 1), (2, 0.5, 15, 1), (64, 1 / 64, 12, 3)])
+def test_colored_fixed_planes(gpu, monkeypatch, fixed, P, T, S, M):
     # the 1024-spin square lattice at 256 threads and two replicas per
     # workgroup runs the fixed-plane kernel (static LDS planes of 1024 words,
     # both replicas unrolled); PIQMC_FIXED_PLANES=0 runs the general cached
-    # kernel on the same tile: both equal the oracle bit for bit, with traces
+    # kernel on the same tile: both equal the oracle bit for bit, with traces,
+    # also with several sweeps per field step
     monkeypatch.setenv("PIQMC_FIXED_PLANES", fixed)
     N, nbs, _, _, _ = pm_j_problem(32, seed=5 + P)
-    pb, got, ref, etg, etr = _run(gpu, nbs, P, np.linspace(3.0, 1e-8, S), 1, T, R=4, seed=17 + P,
+    pb, got, ref, etg, etr = _run(gpu, nbs, P, np.linspace(3.0, 1e-8, S), M, T, R=4, seed=17 + P,
                                   rbase=3, sbase=5, block=256, rpw=2)
     assert pb.pm_j and pb.ncolors == 2 and N == 1024
     assert np.array_equal(got, ref)
