@@ -306,6 +306,13 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
             }
     }
 
+    // A queue drain's accepted bits into their word, as an LDS atomic XOR with no
+    // return: a drain holds each word once and no other wavefront writes it in
+    // the phase, so this equals a read-modify-write without its round trip.
+    auto xor_word = [](WT *w, WT v) __attribute__((always_inline)) {
+        if constexpr (sizeof(WT) == 4) atomicXor(reinterpret_cast<unsigned int *>(w), (unsigned int)v);
+        else *w ^= v;
+    };
     // a neighbour's word: cached neighbours are byte offsets (no shift per word)
     auto nbword = [](const WT *base, int nb) __attribute__((always_inline)) -> WT {
         if constexpr (CACHED) return *reinterpret_cast<const WT *>(reinterpret_cast<const char *>(base) + nb);
@@ -407,7 +414,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                                     und &= ~(Rb ^ tb);
                                 }
                             }
-                            st[((size_t)rr * 2 + tau) * PW + i] ^= lt;
+                            xor_word(&st[((size_t)rr * 2 + tau) * PW + i], lt);
                         }
                         q2 -= cnt;
                         __builtin_amdgcn_wave_barrier();
@@ -438,7 +445,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                                 und &= ~(Rb ^ tb);
                             }
                             if (und == 0) {
-                                st[((size_t)rr * 2 + tau) * PW + i] ^= lt;
+                                xor_word(&st[((size_t)rr * 2 + tau) * PW + i], lt);
                             } else {
                                 more = true;
                                 rec2 = make_uint4(r2.x, r2.y, (uint32_t)lt, (uint32_t)und);
@@ -497,7 +504,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                                     und &= ~(Rb ^ tb);
                                 }
                             }
-                            st[((size_t)rr * 2 + tau) * PW + i] ^= lt;
+                            xor_word(&st[((size_t)rr * 2 + tau) * PW + i], lt);
                         }
                         qn = 0;
                         q2 = 0;
@@ -534,7 +541,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                                     und &= ~(Rb ^ tb);
                                 }
                             }
-                            st[((size_t)rr * 2 + tau) * PW + i] ^= lt;
+                            xor_word(&st[((size_t)rr * 2 + tau) * PW + i], lt);
                         }
                         qn -= cnt;
                         __builtin_amdgcn_wave_barrier();
