@@ -306,6 +306,27 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
             }
     }
 
+    // Queue records' first word: index | replica << 16 | parity << 31, or in the
+    // fixed-plane sweep the word's byte offset in the planes, index << 2 |
+    // parity << 12 | replica << 13, so that a drain needs no address arithmetic
+    // to write the word back.
+    static_assert(kFixPlane * 4 == (1 << 12) && kFixRep == 2, "fixed-plane record layout");
+    auto rec_i = [](uint32_t x) __attribute__((always_inline)) -> int {
+        if constexpr (FIXED) return (int)((x >> 2) & (uint32_t)(kFixPlane - 1));
+        else return (int)(x & 0xffffu);
+    };
+    auto rec_rr = [](uint32_t x) __attribute__((always_inline)) -> int {
+        if constexpr (FIXED) return (int)(x >> 13);
+        else return (int)((x >> 16) & 0x7fffu);
+    };
+    auto rec_tau = [](uint32_t x) __attribute__((always_inline)) -> int {
+        if constexpr (FIXED) return (int)((x >> 12) & 1u);
+        else return (int)(x >> 31);
+    };
+    auto rec_word = [&](uint32_t x) __attribute__((always_inline)) -> WT * {
+        if constexpr (FIXED) return reinterpret_cast<WT *>(reinterpret_cast<unsigned char *>(st) + x);
+        else return &st[((size_t)rec_rr(x) * 2 + rec_tau(x)) * PW + rec_i(x)];
+    };
     // A queue drain's accepted bits into their word, as an LDS atomic XOR with no
     // return: a drain holds each word once and no other wavefront writes it in
     // the phase, so this equals a read-modify-write without its round trip.
@@ -398,8 +419,8 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                         const int lane = tid & 63;
                         if (lane < cnt) {
                             const uint4 rec = wq2[q2 - cnt + lane];
-                            const int i = (int)(rec.x & 0xffffu), rr = (int)((rec.x >> 16) & 0x7fffu);
-                            const int tau = (int)(rec.x >> 31);  // the word's Trotter parity
+                            const int i = rec_i(rec.x), rr = rec_rr(rec.x);
+                            const int tau = rec_tau(rec.x);  // the word's Trotter parity
                             const WT m0 = (WT)rec.y;
                             WT lt = (WT)rec.z, und = (WT)rec.w;
                             const uint32_t rep = a.replica_base + (uint32_t)(r0 + rr);
@@ -414,7 +435,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                                     und &= ~(Rb ^ tb);
                                 }
                             }
-                            xor_word(&st[((size_t)rr * 2 + tau) * PW + i], lt);
+                            xor_word(rec_word(rec.x), lt);
                         }
                         q2 -= cnt;
                         __builtin_amdgcn_wave_barrier();
@@ -430,8 +451,8 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                         uint4 rec2 = make_uint4(0u, 0u, 0u, 0u);
                         if (lane < cnt) {
                             const uint2 r2 = wq1[qn - cnt + lane];
-                            const int i = (int)(r2.x & 0xffffu), rr = (int)((r2.x >> 16) & 0x7fffu);
-                            const int tau = (int)(r2.x >> 31);
+                            const int i = rec_i(r2.x), rr = rec_rr(r2.x);
+                            const int tau = rec_tau(r2.x);
                             const WT m0 = (WT)r2.y;
                             WT lt = 0, und = m0;
                             uint32_t out[4];
@@ -445,7 +466,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                                 und &= ~(Rb ^ tb);
                             }
                             if (und == 0) {
-                                xor_word(&st[((size_t)rr * 2 + tau) * PW + i], lt);
+                                xor_word(rec_word(r2.x), lt);
                             } else {
                                 more = true;
                                 rec2 = make_uint4(r2.x, r2.y, (uint32_t)lt, (uint32_t)und);
@@ -472,21 +493,24 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                         const int lane = tid & 63;
                         if (lane < qn + q2) {
                             int i, rr, qd0, tau;
+                            uint32_t rx;
                             WT m0, lt, und;
                             if (lane < qn) {
                                 const uint2 r2 = wq1[lane];
-                                i = (int)(r2.x & 0xffffu);
-                                rr = (int)((r2.x >> 16) & 0x7fffu);
-                                tau = (int)(r2.x >> 31);
+                                rx = r2.x;
+                                i = rec_i(r2.x);
+                                rr = rec_rr(r2.x);
+                                tau = rec_tau(r2.x);
                                 m0 = (WT)r2.y;
                                 lt = 0;
                                 und = m0;
                                 qd0 = 0;
                             } else {
                                 const uint4 rec = wq2[lane - qn];
-                                i = (int)(rec.x & 0xffffu);
-                                rr = (int)((rec.x >> 16) & 0x7fffu);
-                                tau = (int)(rec.x >> 31);
+                                rx = rec.x;
+                                i = rec_i(rec.x);
+                                rr = rec_rr(rec.x);
+                                tau = rec_tau(rec.x);
                                 m0 = (WT)rec.y;
                                 lt = (WT)rec.z;
                                 und = (WT)rec.w;
@@ -504,7 +528,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                                     und &= ~(Rb ^ tb);
                                 }
                             }
-                            xor_word(&st[((size_t)rr * 2 + tau) * PW + i], lt);
+                            xor_word(rec_word(rx), lt);
                         }
                         qn = 0;
                         q2 = 0;
@@ -522,8 +546,8 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                             } else {
                                 rec = wq[qn - cnt + lane];
                             }
-                            const int i = (int)(rec.x & 0xffffu), rr = (int)((rec.x >> 16) & 0x7fffu);
-                            const int tau = (int)(rec.x >> 31);
+                            const int i = rec_i(rec.x), rr = rec_rr(rec.x);
+                            const int tau = rec_tau(rec.x);
                             const WT m0 = (WT)rec.y, m1 = NS() > 1 ? (WT)rec.z : WT(0), m2 = NS() > 2 ? (WT)rec.w : WT(0);
                             const uint32_t rep = a.replica_base + (uint32_t)(r0 + rr);
                             WT lt = 0, und = m0 | m1 | m2;
@@ -541,7 +565,7 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                                     und &= ~(Rb ^ tb);
                                 }
                             }
-                            xor_word(&st[((size_t)rr * 2 + tau) * PW + i], lt);
+                            xor_word(rec_word(rec.x), lt);
                         }
                         qn -= cnt;
                         __builtin_amdgcn_wave_barrier();
@@ -862,9 +886,10 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
                                           // formed here: hoisted out of the step loop, the
                                           // records' fixed parts would hold registers
-                                          uint32_t rx = (uint32_t)i;
+                                          uint32_t rx = FIXED ? (uint32_t)i << 2 : (uint32_t)i;
                                           asm volatile("" : "+v"(rx));
-                                          rx |= ((uint32_t)rr << 16) | ((uint32_t)tau << 31);
+                                          if constexpr (FIXED) rx |= ((uint32_t)tau << 12) | ((uint32_t)rr << 13);
+                                          else rx |= ((uint32_t)rr << 16) | ((uint32_t)tau << 31);
                                           if (SINGLE)
                                               wq1[pos] = make_uint2(rx, (uint32_t)qm0);
                                           else
