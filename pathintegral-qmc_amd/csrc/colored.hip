@@ -680,7 +680,10 @@ __device__ void colored_body(const ColArgs<WT> &a, int r0, int nrep_, unsigned c
     #pragma unroll
                                         for (int kk = 0; kk < KMAX; ++kk) ge[kk] = gek[kk];
                                         ge[KMAX] = WT(0);
-                                        flips = (e0 & ge[ka0]) | (e1 & ge[ka1]) | (e2 & ge[ka2]);
+                                        // fixed planes (degree 4, no fields): one antiparallel
+                                        // Trotter bond cancels J_perp, so that class accepts from
+                                        // count 2 in every step with T > 0 (the host checks it)
+                                        flips = (e0 & ge[ka0]) | (e1 & ge[FIXED ? 2 : ka1]) | (e2 & ge[ka2]);
                                         if constexpr (SINGLE) {
                                             // the step's one stochastic class (u_t, u_s): exactly the
                                             // bits of Trotter class u_t with count u_s (no accepted
@@ -1182,7 +1185,8 @@ inline size_t queue_lds(int N, int rpw, int word_bytes, int block)
 }
 
 template <typename WT, int ZU>
-void launch_uniform(const ColArgs<WT> &a, const piqmc_problem *pb, int grid, int block, size_t lds, hipStream_t st)
+void launch_uniform(const ColArgs<WT> &a, const piqmc_problem *pb, int grid, int block, size_t lds, hipStream_t st,
+                    bool ka1_two = false)
 {
     int biggest = 0;
     for (int c = 0; c < pb->ncolors; ++c) biggest = std::max(biggest, pb->color_off[c + 1] - pb->color_off[c]);
@@ -1191,7 +1195,7 @@ void launch_uniform(const ColArgs<WT> &a, const piqmc_problem *pb, int grid, int
     if constexpr (sizeof(WT) == 4) {
         // fixed planes: two replicas per workgroup (R even), both colors
         // exactly two spins per thread, N within a plane
-        const bool fixed = ZU == 4 && a.rpw == kFixRep && block == (int)kFixBlock &&
+        const bool fixed = ZU == 4 && ka1_two && a.rpw == kFixRep && block == (int)kFixBlock &&
                            fixed_tile_applies(pb, a.R, sizeof(WT));
         if (fixed && queue_enabled()) {
             launch_uniform_q<WT, ZU, true>(a, cached, grid, block, 0, st, true);
@@ -2664,7 +2668,11 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
         }
     } else if (pmj) {
         if (a.zuni == 4) {
-            launch_uniform<WT, 4>(a, pb, grid, block, lds, st);
+            // every step's accept threshold for one antiparallel Trotter bond
+            // is count >= 2 (the fixed-plane kernel takes it as a constant)
+            bool ka1_two = true;
+            for (int s = 0; s < S; ++s) ka1_two = ka1_two && ((ztab[(size_t)s * 8 + 4] >> 8) & 15u) == 2u;
+            launch_uniform<WT, 4>(a, pb, grid, block, lds, st, ka1_two);
         } else if (a.zuni == 6) {
             launch_uniform<WT, 6>(a, pb, grid, block, lds, st);
         } else if (a.zuni >= 0) {
