@@ -2598,6 +2598,20 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
     }
     std::vector<uint32_t> thr, ztab;
     const bool pmj = pb->pmj && pmj_tables(pb, S, jp, temps, thr, ztab);
+    if (rpw <= 0 && pb->pmjf && !pb->pmj && pb->ncolors == 2 && pb->zuni_nf == 6 && cached_fields_enabled()) {
+        // the cached field sweep evaluates each step's decision classes once per
+        // workgroup for all of its replicas: as many replicas per workgroup as
+        // put every workgroup on the device at once (one 512-thread workgroup
+        // per CU), within the LDS (3-D L=12 field workload, 1024 replicas: 4
+        // per workgroup 251 ms, 2 298 ms, 3 400 ms, 1 392 ms)
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+            ncu = 256;
+        const size_t per = sizeof(WT) * 2 * (size_t)N, fixed_part = 4 * (kClassStride + 8) + 16 * sizeof(double);
+        const int lds_max = (int)std::max<size_t>(1, (160 * 1024 - fixed_part) / per);
+        rpw = std::max(1, std::min(lds_max, (R + ncu - 1) / ncu));
+    }
     if (rpw <= 0 && block <= 0 && fixed_tile_applies(pb, R, sizeof(WT))) {
         // the fixed-plane kernel's tile (1024-spin workload: 9.25 ms against
         // 9.6 ms for the best general tile)
