@@ -16,6 +16,15 @@ if not os.path.exists(LIB_PATH):
         "piqmc: %s is not built; run `python pathintegral-qmc_amd/build_lib.py` "
         "(or __graft_entry__.build())" % LIB_PATH)
 
+# One HIP runtime per process: torch's, where torch is installed.  Loaded
+# first, torch's runtime is the one the library binds to; loaded after the
+# library, torch would find the system runtime the library pulled in and see
+# no GPU ("No HIP GPUs are available").
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 lib = C.CDLL(LIB_PATH)
 
 _vp = C.c_void_p

@@ -109,3 +109,25 @@ def test_quantum_anneal_parallel_dropin(gpu):
     E1 = [sa.ClassicalIsingEnergy(confs[:, k], J) for k in range(20)]
     assert set(np.unique(confs)) <= {-1.0, 1.0}
     assert np.mean(E1) < E0 and min(E1) / N < -1.2
+
+
+def test_importing_piqmc_before_torch_keeps_the_gpu(gpu):
+    # a fresh process that imports the package first and torch after it: the
+    # package brings in torch's HIP runtime before loading its library, so
+    # torch still sees the GPU and a device sweep runs
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from piqmc import device, instances\n"
+            "import numpy as np, torch\n"
+            "assert torch.cuda.is_available()\n"
+            "N, r, c, v = instances.lattice_pm_j(8, 2, np.random.RandomState(1))\n"
+            "pb = device.Problem(instances.neighbors(N, r, c, v))\n"
+            "x = torch.ones((2, 8, N), dtype=torch.int8, device='cuda')\n"
+            "pb.qa_colored(x, np.linspace(3.0, 1e-8, 5), 1, 0.1, 7)\n"
+            "torch.cuda.synchronize()\n"
+            "print('ok')\n") % os.path.join(root, "pathintegral-qmc_amd")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
+    assert out.returncode == 0 and out.stdout.strip().endswith("ok"), out.stderr[-2000:]
