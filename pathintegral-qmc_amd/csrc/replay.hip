@@ -50,17 +50,45 @@ __device__ __forceinline__ uint32_t getbit(const uint32_t *reg, int Ls, int lane
 // row of the bit-packed spins spans a multiple of 32 bytes.
 __host__ __device__ inline int replay_stride(int L) { return L < 8 ? 8 : L; }
 
-// LDS image of a workgroup, words slot-interleaved with stride Ls:
-//   spin regions [nreg][Wn + 1][Ls] u32 (row Wn of each region stays zero),
-//   rand() rings [31][Ls] u32, and with STAGED (8-byte aligned) the neighbour
-//   rows [N][Zp] int2, padded with zero fields to a multiple of 4 entries,
-//   then the sweep's permutations [N][Ls] u16.
-__host__ __device__ inline size_t replay_words(int nreg, int Wn, int Ls)
+// Words of one spin region: (Wn + 1) rows of Ls words, rounded up to a power
+// of two, so that regions aligned to their size hold every row offset below
+// the alignment (see ORA in replay_kernel).
+__host__ __device__ inline int replay_region_words(int Wn, int Ls)
 {
-    return ((size_t)(nreg * (Wn + 1) + 31) * Ls + 1) & ~(size_t)1;
+    int rs = 1;
+    while (rs < (Wn + 1) * Ls) rs <<= 1;
+    return rs;
 }
 
-template <bool F64J, bool STAGED, bool SA>
+// LDS image of a workgroup, words slot-interleaved with stride Ls: up to one
+// region of padding (the regions start aligned to their size), spin regions
+// [nreg][RS] u32 of which rows [Wn + 1][Ls] are used (row Wn stays zero),
+// rand() rings [31][Ls] u32, and with STAGED (8-byte aligned) the neighbour
+// rows [N][Zp] int2, padded with zero fields to a multiple of 4 entries, then
+// the sweep's permutations [N][Ls] u16.
+__host__ __device__ inline size_t replay_words(int nreg, int Wn, int Ls)
+{
+    const size_t rs = (size_t)replay_region_words(Wn, Ls);
+    return (rs + nreg * rs + (size_t)31 * Ls + 1) & ~(size_t)1;
+}
+
+// LDS byte address of a pointer into shared memory, and a word read at one.
+typedef __attribute__((address_space(3))) const uint32_t LdsWord;
+__device__ __forceinline__ uint32_t lds_addr(const void *p)
+{
+    return (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const char *)p;
+}
+__device__ __forceinline__ uint32_t lds_word(uint32_t addr)
+{
+    return *(LdsWord *)(uintptr_t)addr;
+}
+
+// ORA (staged float couplings, at most 8 slots per workgroup): a neighbour's
+// word is at (region-aligned base + slot * 4) + (row offset & ~31), and the
+// two never share a bit (the slot offset is below 32, the row offset below the
+// region's alignment), so one AND-OR forms the address instead of an AND and
+// an add.
+template <bool F64J, bool STAGED, bool SA, bool ORA = false>
 __global__ void replay_kernel(ReplayArgs a)
 {
     extern __shared__ uint32_t sm[];
@@ -75,7 +103,6 @@ __global__ void replay_kernel(ReplayArgs a)
     const int r = blockIdx.x * L + lane;
     const int N = a.N, P = a.P, Z = a.Z, Wn = a.Wn, R = a.R;
     const int nreg = SA ? 1 : (P == 2 ? 2 : 3);
-    const int Wr = Wn + 1;             // rows per spin region (the last one zero)
     const int Zr = STAGED ? a.Zp : Z;  // row stride of the table in use
     int2 *lrows = reinterpret_cast<int2 *>(sm + replay_words(nreg, Wn, Ls));
     uint16_t *lperm = reinterpret_cast<uint16_t *>(lrows + (size_t)N * a.Zp);
@@ -102,16 +129,20 @@ __global__ void replay_kernel(ReplayArgs a)
         __syncthreads();
     }
     if (ln >= Lw || r >= a.R) return;  // no barriers below: a slot touches only its own words
-    uint32_t *cur = sm, *s1, *sp, *ring;
+    // regions aligned to their size (RS words)
+    const int RS = replay_region_words(Wn, Ls);
+    const uint32_t rsb = (uint32_t)RS * 4u;
+    uint32_t *base = sm + ((rsb - (lds_addr(sm) & (rsb - 1u))) & (rsb - 1u)) / 4u;
+    uint32_t *cur = base, *s1, *sp, *ring;
     if (SA) {
         s1 = sp = cur;
-        ring = sm + Wr * Ls;
+        ring = base + RS;
     } else {
-        s1 = sm + Wr * Ls;
-        sp = (P == 2) ? s1 : sm + 2 * Wr * Ls;
-        ring = sm + nreg * Wr * Ls;  // nreg spin regions, then the rand() rings
+        s1 = base + RS;
+        sp = (P == 2) ? s1 : base + 2 * RS;
+        ring = base + nreg * RS;  // nreg spin regions, then the rand() rings
     }
-    for (int g = 0; g < nreg; ++g) sm[(g * Wr + Wn) * Ls + lane] = 0u;  // zero rows
+    for (int g = 0; g < nreg; ++g) base[g * RS + Wn * Ls + lane] = 0u;  // zero rows
     auto load = [&](uint32_t *reg, int k) {
         for (int w = 0; w < Wn; ++w) reg[w * Ls + lane] = a.bits[((size_t)k * Wn + w) * R + r];
     };
@@ -156,6 +187,7 @@ __global__ void replay_kernel(ReplayArgs a)
                 if (own) load(cur, k);
                 // this slot's word 0 in the slice's region
                 const char *regb = reinterpret_cast<const char *>(reg + lane);
+                const uint32_t rega = lds_addr(reg + lane);
                 float e = 0.0f;
                 int inext = STAGED ? (int)lperm[lane] : (int)pg[0];
                 // the first four row entries of the next attempt's spin, read
@@ -221,7 +253,8 @@ __global__ void replay_kernel(ReplayArgs a)
                             uint32_t wn[4];
 #pragma unroll
                             for (int q = 0; q < 4; ++q)
-                                wn[q] = *reinterpret_cast<const uint32_t *>(regb + ((uint32_t)rw[q].x & ~31u));
+                                wn[q] = ORA ? lds_word(((uint32_t)rw[q].x & ~31u) | rega)
+                                            : *reinterpret_cast<const uint32_t *>(regb + ((uint32_t)rw[q].x & ~31u));
                             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                             for (int q = 0; q < 4; ++q) {
@@ -412,10 +445,13 @@ int run_replay(const piqmc_problem *pb, int R, int P, int S, int M, float temp, 
     a.draws = d_draws;
     a.ties = ties;
     a.jd = pb->dense ? pb->d_jd : nullptr;
+    // the AND-OR neighbour address needs slot offsets below 32 bytes
+    const bool ora = staged && !pb->dense && L <= 8;
     auto pick = [&](auto SAc) {
         constexpr bool S_ = decltype(SAc)::value;
         return pb->dense ? (staged ? replay_kernel<true, true, S_> : replay_kernel<true, false, S_>)
-                         : (staged ? replay_kernel<false, true, S_> : replay_kernel<false, false, S_>);
+                         : (staged ? (ora ? replay_kernel<false, true, S_, true> : replay_kernel<false, true, S_>)
+                                   : replay_kernel<false, false, S_>);
     };
     auto kern = sa ? pick(std::true_type()) : pick(std::false_type());
     (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds(L));
