@@ -369,6 +369,17 @@ def test_colored_fixed_planes(gpu, monkeypatch, fixed, P, T, S, M):
     assert np.array_equal(etg, etr)
 
 
+def test_fixed_tile_with_odd_replica_count(gpu):
+    # 256 threads and two replicas per workgroup with an odd replica count: the
+    # last workgroup holds one replica, so the general cached kernel runs on
+    # that tile instead of the fixed-plane one; the oracle's spins either way
+    N, nbs, _, _, _ = pm_j_problem(32, seed=41)
+    pb, got, ref, etg, etr = _run(gpu, nbs, 64, np.linspace(3.0, 1e-8, 20), 1, 1 / 64, R=3, seed=43,
+                                  rbase=1, sbase=2, block=256, rpw=2)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(etg, etr)
+
+
 @pytest.mark.parametrize("L,dim,P", [(8, 2, 64), (4, 2, 2), (4, 3, 128), (6, 2, 20), (12, 3, 66)])
 @pytest.mark.parametrize("offset", [0, 1])
 def test_colored_pack_roundtrip(gpu, L, dim, P, offset):
