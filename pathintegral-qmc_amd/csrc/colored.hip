@@ -2127,8 +2127,10 @@ __device__ void dense_fields_replica(const ColArgs<WT> &a, int r, unsigned char 
 // the word, not the slice), so the decisions equal the one-wavefront sweep's.
 // Used where one replica's tables fill the LDS and the other wavefronts of the
 // workgroup would idle: the chain of N decisions per half sweep, which sets a
-// lone replica's time, is then shared out.
-template <typename WT>
+// lone replica's time, is then shared out.  FL: flipped slices per update pass,
+// at most the wavefront's slices (W / nw), so a pass issues no reads for slots
+// that can never hold a flip.
+template <typename WT, int FL = 4>
 __device__ void dense_fields_split(const ColArgs<WT> &a, int r, unsigned char *base, int wv, int nw)
 {
     const int N = a.N, W = a.W, Z = a.Z, lane = threadIdx.x & 63, NP = fields_pitch(N);
@@ -2181,7 +2183,7 @@ __device__ void dense_fields_split(const ColArgs<WT> &a, int r, unsigned char *b
                     // the ballot is over lanes: lane l decided slice s0 + l
                     const WT bal = dense_decide<WT, true>(a, h, S_, O, sl, tau, jp, Ts, g, i, rep) << s0;
                     if (lane == 0 && bal) atomicXor(&mine[i], bal);
-                    constexpr int kFl = 4;
+                    constexpr int kFl = FL;
                     int jj[kPre];
                     double jv[kPre];
 #pragma unroll
@@ -2258,11 +2260,22 @@ __global__ void __launch_bounds__(256) dense_fields_kernel(ColArgs<WT> a)
 }
 
 // One replica per workgroup, its slices split over the workgroup's wavefronts.
-template <typename WT>
+template <typename WT, int FL>
 __global__ void __launch_bounds__(512) dense_fields_split_kernel(ColArgs<WT> a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    dense_fields_split<WT>(a, blockIdx.x, smem, threadIdx.x >> 6, blockDim.x >> 6);
+    dense_fields_split<WT, FL>(a, blockIdx.x, smem, threadIdx.x >> 6, blockDim.x >> 6);
+}
+
+// Flipped slices per update pass of the split sweep for a wavefront's slices.
+template <typename WT>
+const void *fields_split_kernel_for(int slices)
+{
+    if constexpr (sizeof(WT) == 4) {
+        if (slices <= 1) return (const void *)dense_fields_split_kernel<WT, 1>;
+        if (slices <= 2) return (const void *)dense_fields_split_kernel<WT, 2>;
+    }
+    return (const void *)dense_fields_split_kernel<WT, 4>;
 }
 
 // int8 [R][P][N] -> packed [R][2][N] (bit m of parity tau = slice 2m+tau, set = -1)
@@ -2569,9 +2582,10 @@ int run_colored(const piqmc_problem *pb, int R, int P, int S, int M,
             const int ns = fields_split_waves(W, wf);
             if (ns > 1) {
                 // one replica's tables fill the LDS: its slices over ns wavefronts
-                (void)hipFuncSetAttribute((const void *)dense_fields_split_kernel<WT>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)per);
-                dense_fields_split_kernel<WT><<<R, 64 * ns, per, st>>>(a);
+                const void *kf = fields_split_kernel_for<WT>(W / ns);
+                (void)hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)per);
+                void *kargs[] = {(void *)&a};
+                PIQMC_HIP(hipLaunchKernel(kf, dim3(R), dim3(64 * ns), kargs, per, st));
             } else {
                 (void)hipFuncSetAttribute((const void *)dense_fields_kernel<WT>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)(per * wf));

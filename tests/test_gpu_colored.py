@@ -488,17 +488,20 @@ def test_autotuned_batch_equals_individual_runs(gpu):
         assert torch.equal(c8, ref), t
 
 
+@pytest.mark.parametrize("P", [128, 32, 16])
 @pytest.mark.parametrize("split", ["1", "0"])
-def test_cached_fields_split_over_wavefronts(gpu, monkeypatch, split):
-    # 256-spin SK at P=128: one replica's field tables fill a workgroup's LDS,
-    # so its 64 slices of each parity are shared out over four wavefronts
-    # (PIQMC_FIELDS_SPLIT=0: one wavefront per replica); both give the CPU
+def test_cached_fields_split_over_wavefronts(gpu, monkeypatch, split, P):
+    # 256-spin SK: one replica's field tables fill a workgroup's LDS (P=128),
+    # or a lone replica's chain is shared out anyway, so the P/2 slices of
+    # each parity go over eight wavefronts: 8, 2 and 1 slices each, the last
+    # two with update passes of two and one flipped slices
+    # (PIQMC_FIELDS_SPLIT=0: one wavefront per replica); all give the CPU
     # twin's spins and energies
     from piqmc import instances
     monkeypatch.setenv("PIQMC_FIELDS_SPLIT", split)
     n, r, c, v = instances.sk(256, np.random.RandomState(4))
     nbs = instances.neighbors(n, r, c, v)
-    pb, got, ref, etg, etr = _run(gpu, nbs, 128, np.linspace(3.0, 1e-8, 6), 1, 1 / 128, R=2, seed=17,
+    pb, got, ref, etg, etr = _run(gpu, nbs, P, np.linspace(3.0, 1e-8, 6), 1, 1 / P, R=2, seed=17,
                                   rbase=3, sbase=1)
     assert pb.exact_rows
     assert np.array_equal(got, ref)
